@@ -1,0 +1,157 @@
+/*
+ * csipm.h -- C ABI of libcsipm.so, the MI355X (gfx950) implementation of the
+ * matrix-free interior-point hot path of CompressedSensingIPM.jl.
+ *
+ * Every vector argument is a DEVICE pointer owned by the caller (a Julia
+ * ROCArray under GC.@preserve, a torch tensor, hipMalloc memory).  Calls are
+ * asynchronous on the handle's stream unless stated otherwise.  A handle owns
+ * only its internal state (FFT twiddles, the sampling mask, scratch spectra,
+ * CG scalars) and is not re-entrant: like the reference's FFTOperator buffers
+ * (src/fft_utils.jl:16-54), its scratch is overwritten by every apply.
+ *
+ * Layout: Julia column-major, dimension 1 contiguous.  beta, z and the real
+ * signal have N = N1*N2*N3 entries.  A spectrum ("v") is complex interleaved
+ * (re, im) with dimension 1 of length M1+1 (rdft) or N1 (full), M1 = N1/2.
+ * "T" below is double for CSIPM_F64 handles and float for CSIPM_F32 handles;
+ * the beta/DFT maps always take double.
+ *
+ * Status: 0 on success; otherwise a CSIPM_ERR_* code with a message available
+ * from csipm_last_error() (thread-local).
+ */
+#ifndef CSIPM_H
+#define CSIPM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  CSIPM_OK = 0,
+  CSIPM_ERR_ARG = 1,          /* bad argument (null pointer, bad enum)      */
+  CSIPM_ERR_SHAPE = 2,        /* sizes / lengths inconsistent               */
+  CSIPM_ERR_ALLOC = 3,        /* device allocation failed                   */
+  CSIPM_ERR_HIP = 4,          /* HIP runtime error                          */
+  CSIPM_ERR_UNSUPPORTED = 5,  /* size or option not implemented            */
+  CSIPM_ERR_NOT_CONVERGED = 6 /* iterative method stopped without success */
+};
+
+enum { CSIPM_F64 = 0, CSIPM_F32 = 1 };
+
+typedef struct csipm_op* csipm_op_t;
+
+/* Version string of the library. */
+const char* csipm_version(void);
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* csipm_last_error(void);
+
+/* Create an operator handle.  Replaces FFTOperator{VT}(nβ, DFTdim, DFTsize,
+ * index_missing, rdft) and its plan_rfft (src/fft_utils.jl:16-54).
+ *   dim        1, 2 or 3 (FFTParameters.DFTdim, src/fft_utils.jl:1-14)
+ *   sizes      N1, N2, N3 (unused trailing entries ignored)
+ *   rdft       must be 1 (the real-to-complex path); 0 is reserved
+ *   precision  CSIPM_F64 or CSIPM_F32: arithmetic of the operator apply
+ *   missing    HOST array of n_missing 0-based linear indices (index_missing
+ *              of FFTParameters, converted from Julia's 1-based Int), may be
+ *              NULL when n_missing == 0
+ *   stream     hipStream_t to run on (NULL = default stream)            */
+int csipm_op_create(csipm_op_t* h, int dim, const int64_t sizes[3], int rdft,
+                    int precision, const int64_t* missing, int64_t n_missing,
+                    void* stream);
+int csipm_op_destroy(csipm_op_t h);
+/* Change the stream later calls on h are issued to. */
+int csipm_op_set_stream(csipm_op_t h, void* stream);
+/* Query N (length of beta) and Nc (complex entries of the half spectrum). */
+int csipm_op_info(csipm_op_t h, int64_t* n, int64_t* nc, int* precision);
+
+/* beta -> DFT map.  beta_to_DFT! (src/fft_utils.jl:195-204, per-element form
+ * src/mapping_cpu_kernels.jl:353-738).  v: (M1+1)*N2*N3 complex when full==0,
+ * N1*N2*N3 complex when full==1.  Bit-exact for even sizes.               */
+int csipm_beta_to_dft(csipm_op_t h, const double* beta, double* v, int full);
+/* DFT -> beta map.  DFT_to_beta! (src/fft_utils.jl:144-153, per-element
+ * form src/mapping_cpu_kernels.jl:1-351); v laid out as above.          */
+int csipm_dft_to_beta(csipm_op_t h, const double* v, double* beta, int full);
+
+/* y = M beta: unitary inverse DFT of beta_to_DFT(beta), zeroed at missing
+ * samples.  M_perp_beta (src/fft_utils.jl:95-117).  T* in/out, N each.   */
+int csipm_m_perp_beta(csipm_op_t h, const void* beta, void* y);
+/* beta = M^T z = DFT_to_beta(R2C(z)/sqrt(N)).  M_perpt_z (src/fft_utils.jl:73-93). */
+int csipm_m_perpt_z(csipm_op_t h, const void* z, void* beta);
+/* y = M^T M x.  M_perpt_M_perp_vec (src/fft_utils.jl:119-123).          */
+int csipm_mtm(csipm_op_t h, const void* x, void* y);
+
+/* y = alpha*K*x + beta*y with K = [[M^T M + L1, L2], [L2, L1]] on 2N-vectors.
+ * CondensedFFTKKTSystem mul! (src/fft_kkt.jl:27-44).                    */
+int csipm_kkt_apply(csipm_op_t h, const void* x, void* y, const void* L1,
+                    const void* L2, double alpha, double beta);
+
+/* Preconditioner data from the slack diagonals sigma1, sigma2 (N each):
+ * L1 = s1+s2, L2 = s1-s2 and P = [[1+L1, L2], [L2, L1]]^-1, i.e. the
+ * reference's Minv = 1/(1+L1), P22 = S = 1/(L1 - L2^2 Minv), P12 = -L2 S Minv,
+ * P11 = Minv + Minv L2 S L2 Minv, evaluated in the equivalent closed form
+ * P11 = L1/det, P12 = -L2/det, P22 = (1+L1)/det, det = L1 + 4 s1 s2, which
+ * avoids the cancellation in L1 - L2^2 Minv.
+ * build_kkt! (src/fft_kkt.jl:285-307).  All T*.                          */
+int csipm_precond_build(csipm_op_t h, const void* sigma1, const void* sigma2,
+                        void* L1, void* L2, void* P11, void* P12, void* P22);
+/* y = alpha*P*x + beta*y, P = [[P11, P12], [P12, P22]] on 2N-vectors.
+ * FFTPreconditioner mul! (src/fft_kkt.jl:73-85).                          */
+int csipm_precond_apply(csipm_op_t h, const void* x, void* y, const void* P11,
+                        const void* P12, const void* P22, double alpha,
+                        double beta);
+
+/* Preconditioned CG on K x = b (2N), x0 = 0, with Krylov.jl's cg stopping
+ * rule sqrt(r'Pr) <= atol + rtol*sqrt(r0'P r0), itmax (<= 0: 2*2N) and the
+ * zero-curvature exit.  Replaces Krylov.krylov_solve!(workspace, K, b; M=P,
+ * atol, rtol) at src/fft_kkt.jl:339.  Synchronous: returns after the solve.
+ *   iters   number of iterations (host)      resid  final sqrt(r'Pr) (host)
+ *   status  0 solved, 1 itmax, 2 zero curvature (host)                    */
+int csipm_cg_solve(csipm_op_t h, const void* b, void* x, const void* L1,
+                   const void* L2, const void* P11, const void* P12,
+                   const void* P22, double atol, double rtol, int64_t itmax,
+                   int64_t* iters, double* resid, int* status);
+
+/* FFTNLPModel (src/fft_model.jl).  x = [beta; c] (2N, T), Mtz0 = M^T z0 (N, T).
+ * obj   f = 1/2 ||M beta||^2 - beta'Mtz0 + lambda*sum(c)   (:66-75)  synchronous
+ * grad  g = [M^T M beta - Mtz0; lambda]                      (:77-89)
+ * cons  c = [-beta - c; beta - c]                             (:91-101)     */
+int csipm_nlp_obj(csipm_op_t h, const void* x, const void* Mtz0, double lambda, double* f);
+int csipm_nlp_grad(csipm_op_t h, const void* x, const void* Mtz0, double lambda, void* g);
+int csipm_nlp_cons(csipm_op_t h, const void* x, void* c);
+
+/* Full KKT product on 6N vectors [beta; c; s1; s2; y1; y2] without MadNLP's
+ * own diagonal terms (MadNLP._kktmul!): the part FFTKKTSystem's mul! adds
+ * (src/fft_kkt.jl:239-246).  y = alpha*A*x + beta*y.                       */
+int csipm_kkt_mul_full(csipm_op_t h, const void* x, void* y, double alpha, double beta);
+/* J^T y for y = [y1; y2] (2N) -> 4N (src/fft_kkt.jl:253-275).              */
+int csipm_jtprod(csipm_op_t h, const void* y, void* out);
+/* solve_kkt! pieces (src/fft_kkt.jl:309-354): condense the 6N right-hand side
+ * w into b (2N), and unpack the condensed solution x into w in place.      */
+int csipm_condense_rhs(csipm_op_t h, const void* w, const void* sigma1, const void* sigma2,
+                       void* b);
+int csipm_unpack_solution(csipm_op_t h, void* w, const void* x, const void* sigma1,
+                          const void* sigma2);
+
+/* Full solve of FFTNLPModel (src/fft_model.jl:66-101) by the library's
+ * primal-dual interior-point driver on the condensed FFT-KKT system.  The
+ * reference delegates this loop to MadNLP (ipm_solve!,
+ * src/CompressedSensingIPM.jl:31-36).
+ *   z0        device, N doubles (observed samples, zero at missing)
+ *   lambda    l1 weight;  tol  IPM tolerance;  max_iter  IPM iteration cap
+ *   cg_rtol   relative PCG tolerance per Newton system; cg_itmax cap
+ *   beta_out  device, N doubles: the recovered beta
+ *   info      host array of 8 doubles: [objective, ipm iterations, total cg
+ *             iterations, status (0 solved, 1 max_iter, 2 failure),
+ *             final dual residual, final mu, seconds, reserved]
+ * Synchronous.                                                           */
+int csipm_solve(csipm_op_t h, const double* z0, double lambda, double tol,
+                int64_t max_iter, double cg_rtol, int64_t cg_itmax,
+                double* beta_out, double* info);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CSIPM_H */
