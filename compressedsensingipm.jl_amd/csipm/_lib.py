@@ -47,6 +47,9 @@ SIGNATURES = {
     "csipm_jtprod": (_i, [_vp, _vp, _vp]),
     "csipm_condense_rhs": (_i, [_vp, _vp, _vp, _vp, _vp]),
     "csipm_unpack_solution": (_i, [_vp, _vp, _vp, _vp, _vp]),
+    "csipm_kkt_apply_timed": (_i, [_vp, _vp, _vp, _vp, _vp, _i64, ctypes.POINTER(_d),
+                                   ctypes.POINTER(_d), ctypes.POINTER(_i)]),
+    "csipm_kkt_pass_name": (ctypes.c_char_p, [_vp, _i]),
 }
 
 

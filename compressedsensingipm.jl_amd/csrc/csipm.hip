@@ -236,22 +236,38 @@ int launch_dim1(csipm_op* h, const T* beta_in, const T* real_in, T* real_out,
   return CSIPM_OK;
 }
 
-// y = M^T M x (epi kind 0) or the KKT epilogue; the whole fused operator
+// y = M^T M x (epi kind 0) or the KKT epilogue; the whole fused operator.
+// ev (optional): an event is recorded after every pass (per-pass timing).
 template <typename T, int D>
-int apply_mtm(csipm_op* h, const T* x, const Epi<T>& epi, const int* done) {
+int apply_mtm(csipm_op* h, const T* x, const Epi<T>& epi, const int* done, hipEvent_t* ev) {
   const T s = (T)(1.0 / (double)h->N);
-  if (D == 1)
-    return launch_dim1<T, 1, SRC_BETA, DST_BETA, MODE_MTM>(h, x, nullptr, nullptr, epi, s, done);
+  int k = 0;
+  auto mark = [&]() -> int {
+    if (ev) HIPCHK(hipEventRecord(ev[k++], h->stream));
+    return CSIPM_OK;
+  };
+  if (D == 1) {
+    CHK((launch_dim1<T, 1, SRC_BETA, DST_BETA, MODE_MTM>(h, x, nullptr, nullptr, epi, s, done)));
+    return mark();
+  }
   if (D == 2) {
     CHK((launch_lines<T, 2, SRC_BETA, DST_BUF, false>(h, 1, x, epi, done)));
+    CHK(mark());
     CHK((launch_dim1<T, 2, SRC_BUF, DST_BUF, MODE_MTM>(h, nullptr, nullptr, nullptr, epi, s, done)));
-    return launch_lines<T, 2, SRC_BUF, DST_BETA, true>(h, 1, nullptr, epi, done);
+    CHK(mark());
+    CHK((launch_lines<T, 2, SRC_BUF, DST_BETA, true>(h, 1, nullptr, epi, done)));
+    return mark();
   }
   CHK((launch_lines<T, 3, SRC_BETA, DST_BUF, false>(h, 2, x, epi, done)));
+  CHK(mark());
   CHK((launch_lines<T, 3, SRC_BUF, DST_BUF, false>(h, 1, nullptr, epi, done)));
+  CHK(mark());
   CHK((launch_dim1<T, 3, SRC_BUF, DST_BUF, MODE_MTM>(h, nullptr, nullptr, nullptr, epi, s, done)));
+  CHK(mark());
   CHK((launch_lines<T, 3, SRC_BUF, DST_BUF, true>(h, 1, nullptr, epi, done)));
-  return launch_lines<T, 3, SRC_BUF, DST_BETA, true>(h, 2, nullptr, epi, done);
+  CHK(mark());
+  CHK((launch_lines<T, 3, SRC_BUF, DST_BETA, true>(h, 2, nullptr, epi, done)));
+  return mark();
 }
 
 template <typename T, int D>
@@ -285,10 +301,11 @@ int apply_m_perpt_z(csipm_op* h, const T* z, T* beta) {
 }
 
 template <typename T>
-int dispatch_mtm(csipm_op* h, const T* x, const Epi<T>& epi, const int* done) {
-  if (h->dim == 1) return apply_mtm<T, 1>(h, x, epi, done);
-  if (h->dim == 2) return apply_mtm<T, 2>(h, x, epi, done);
-  return apply_mtm<T, 3>(h, x, epi, done);
+int dispatch_mtm(csipm_op* h, const T* x, const Epi<T>& epi, const int* done,
+                 hipEvent_t* ev = nullptr) {
+  if (h->dim == 1) return apply_mtm<T, 1>(h, x, epi, done, ev);
+  if (h->dim == 2) return apply_mtm<T, 2>(h, x, epi, done, ev);
+  return apply_mtm<T, 3>(h, x, epi, done, ev);
 }
 
 int grid_for(int64_t n, int per_thread = 1) {
@@ -300,7 +317,7 @@ int grid_for(int64_t n, int per_thread = 1) {
 
 template <typename T>
 int kkt_apply_t(csipm_op* h, const T* x, T* y, const T* L1, const T* L2, double alpha,
-                double beta, double* dot_part, const int* done) {
+                double beta, double* dot_part, const int* done, hipEvent_t* ev = nullptr) {
   Epi<T> e{};
   e.kind = 1;
   e.xb = x;
@@ -312,7 +329,31 @@ int kkt_apply_t(csipm_op* h, const T* x, T* y, const T* L1, const T* L2, double 
   e.alpha = (T)alpha;
   e.beta = (T)beta;
   e.dot_part = dot_part;
-  return dispatch_mtm<T>(h, x, e, done);
+  return dispatch_mtm<T>(h, x, e, done, ev);
+}
+
+// names and algorithmic HBM bytes of the passes of one K apply
+int kkt_pass_info(csipm_op* h, const char** names, double* bytes) {
+  const double s = (double)h->esize(), N = (double)h->N, Nc = (double)h->Nc;
+  const double mask = h->nmiss > 0 ? std::ceil(N / 8.0) : 0.0;
+  if (h->dim == 1) {
+    names[0] = "dim1_mtm_kkt";
+    bytes[0] = 6 * N * s + mask;   // x_beta, x_z, L1, L2 in; y_beta, y_z out
+    return 1;
+  }
+  const double spec = 2 * Nc * s;   // one pass over the half spectrum
+  if (h->dim == 2) {
+    names[0] = "lines2_inv_from_beta"; bytes[0] = N * s + spec;
+    names[1] = "dim1_c2r_mask_r2c";    bytes[1] = 2 * spec + mask;
+    names[2] = "lines2_fwd_kkt";       bytes[2] = spec + 6 * N * s;
+    return 3;
+  }
+  names[0] = "lines3_inv_from_beta"; bytes[0] = N * s + spec;
+  names[1] = "lines2_inv";           bytes[1] = 2 * spec;
+  names[2] = "dim1_c2r_mask_r2c";    bytes[2] = 2 * spec + mask;
+  names[3] = "lines2_fwd";           bytes[3] = 2 * spec;
+  names[4] = "lines3_fwd_kkt";       bytes[4] = spec + 6 * N * s;
+  return 5;
 }
 
 // number of workgroups of the pass that ends in beta space
@@ -745,6 +786,7 @@ int csipm_solve(csipm_op_t h, const double* z0, double lambda, double tol, int64
 
 #define DISPATCH_T(h, F64EXPR, F32EXPR) ((h)->prec == CSIPM_F64 ? (F64EXPR) : (F32EXPR))
 
+extern "C++" {
 template <typename T>
 static int nlp_obj_t(csipm_op* h, const T* x, const T* mtz0, double lam, double* f) {
   T* y = (T*)h->cg_q;   // N-vector scratch
@@ -769,6 +811,7 @@ static int nlp_obj_t(csipm_op* h, const T* x, const T* mtz0, double lam, double*
   *f = 0.5 * a - b + lam * c;
   return CSIPM_OK;
 }
+}  // extern "C++"
 
 int csipm_nlp_obj(csipm_op_t h, const void* x, const void* Mtz0, double lambda, double* f) {
   if (!h || !x || !Mtz0 || !f) return fail(CSIPM_ERR_ARG, "null argument");
@@ -776,6 +819,7 @@ int csipm_nlp_obj(csipm_op_t h, const void* x, const void* Mtz0, double lambda, 
                     nlp_obj_t<float>(h, (const float*)x, (const float*)Mtz0, lambda, f));
 }
 
+extern "C++" {
 template <typename T>
 static int nlp_grad_t(csipm_op* h, const T* x, const T* mtz0, double lam, T* g) {
   CHK(dispatch_mtm<T>(h, x, epi_store<T>(g), nullptr));
@@ -784,6 +828,7 @@ static int nlp_grad_t(csipm_op* h, const T* x, const T* mtz0, double lam, T* g) 
   HIPCHK(hipGetLastError());
   return CSIPM_OK;
 }
+}  // extern "C++"
 
 int csipm_nlp_grad(csipm_op_t h, const void* x, const void* Mtz0, double lambda, void* g) {
   if (!h || !x || !Mtz0 || !g) return fail(CSIPM_ERR_ARG, "null argument");
@@ -802,6 +847,7 @@ int csipm_nlp_cons(csipm_op_t h, const void* x, void* c) {
   return CSIPM_OK;
 }
 
+extern "C++" {
 template <typename T>
 static int kkt_full_t(csipm_op* h, const T* x, T* y, double alpha, double beta) {
   T* Mb = (T*)h->cg_q;
@@ -811,6 +857,7 @@ static int kkt_full_t(csipm_op* h, const T* x, T* y, double alpha, double beta) 
   HIPCHK(hipGetLastError());
   return CSIPM_OK;
 }
+}  // extern "C++"
 
 int csipm_kkt_mul_full(csipm_op_t h, const void* x, void* y, double alpha, double beta) {
   if (!h || !x || !y) return fail(CSIPM_ERR_ARG, "null argument");
@@ -853,6 +900,49 @@ int csipm_unpack_solution(csipm_op_t h, void* w, const void* x, const void* s1, 
                        (const float*)x, (const float*)s1, (const float*)s2, h->N);
   HIPCHK(hipGetLastError());
   return CSIPM_OK;
+}
+
+
+int csipm_kkt_apply_timed(csipm_op_t h, const void* x, void* y, const void* L1, const void* L2,
+                          int64_t nrep, double* pass_ms, double* pass_bytes, int* npass) {
+  if (!h || !x || !y || !L1 || !L2 || !pass_ms || !npass) return fail(CSIPM_ERR_ARG, "null argument");
+  if (nrep < 1) return fail(CSIPM_ERR_ARG, "nrep < 1");
+  const char* names[8];
+  double bytes[8];
+  const int np = kkt_pass_info(h, names, bytes);
+  hipEvent_t ev[9];
+  for (int i = 0; i <= np; ++i) HIPCHK(hipEventCreate(&ev[i]));
+  std::vector<double> acc(np, 0.0);
+  for (int64_t r = 0; r < nrep; ++r) {
+    HIPCHK(hipEventRecord(ev[0], h->stream));
+    int rc = h->prec == CSIPM_F64
+                 ? kkt_apply_t<double>(h, (const double*)x, (double*)y, (const double*)L1,
+                                       (const double*)L2, 1.0, 0.0, nullptr, nullptr, ev + 1)
+                 : kkt_apply_t<float>(h, (const float*)x, (float*)y, (const float*)L1,
+                                      (const float*)L2, 1.0, 0.0, nullptr, nullptr, ev + 1);
+    if (rc) return rc;
+    HIPCHK(hipEventSynchronize(ev[np]));
+    for (int p = 0; p < np; ++p) {
+      float ms = 0;
+      HIPCHK(hipEventElapsedTime(&ms, ev[p], ev[p + 1]));
+      acc[p] += ms;
+    }
+  }
+  for (int i = 0; i <= np; ++i) (void)hipEventDestroy(ev[i]);
+  for (int p = 0; p < np; ++p) {
+    pass_ms[p] = acc[p] / (double)nrep;
+    if (pass_bytes) pass_bytes[p] = bytes[p];
+  }
+  *npass = np;
+  return CSIPM_OK;
+}
+
+const char* csipm_kkt_pass_name(csipm_op_t h, int pass) {
+  const char* names[8];
+  double bytes[8];
+  if (!h) return "";
+  const int np = kkt_pass_info(h, names, bytes);
+  return (pass >= 0 && pass < np) ? names[pass] : "";
 }
 
 }  // extern "C"

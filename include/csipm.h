@@ -134,6 +134,17 @@ int csipm_condense_rhs(csipm_op_t h, const void* w, const void* sigma1, const vo
 int csipm_unpack_solution(csipm_op_t h, void* w, const void* x, const void* sigma1,
                           const void* sigma2);
 
+/* Per-pass timing of the fused K apply (csipm_kkt_apply with alpha=1,
+ * beta=0), the counterpart of FFTOperator's fft_timer / mapping_timer
+ * (src/fft_utils.jl:24-25): HIP events around every pass, averaged over nrep
+ * applies.  pass_ms / pass_bytes: arrays of >= 8; pass_bytes receives the
+ * algorithmic HBM bytes of each pass (may be NULL).  Synchronous.         */
+int csipm_kkt_apply_timed(csipm_op_t h, const void* x, void* y, const void* L1,
+                          const void* L2, int64_t nrep, double* pass_ms,
+                          double* pass_bytes, int* npass);
+/* Name of pass `pass` of the K apply ("" if out of range). */
+const char* csipm_kkt_pass_name(csipm_op_t h, int pass);
+
 /* Full solve of FFTNLPModel (src/fft_model.jl:66-101) by the library's
  * primal-dual interior-point driver on the condensed FFT-KKT system.  The
  * reference delegates this loop to MadNLP (ipm_solve!,
