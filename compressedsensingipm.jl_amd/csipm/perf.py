@@ -1,0 +1,90 @@
+"""Operator microbenchmark: the fused condensed-KKT apply at a given shape.
+
+    python compressedsensingipm.jl_amd/csipm/perf.py --dims 255 255 255 --dtype f32
+    python compressedsensingipm.jl_amd/csipm/perf.py --copy-mib 1024   (copy calibration)
+
+Prints one JSON line with the per-pass HIP-event times (csipm_kkt_apply_timed),
+the algorithmic bytes of each pass and the achieved fraction of 8 TB/s.  The
+copy mode runs one plain device-to-device copy of known size, the reference
+point used to scale rocprofv3's FETCH_SIZE / WRITE_SIZE counters.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(HERE))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--dims", type=int, nargs="+", default=[255, 255, 255])
+    ap.add_argument("--dtype", choices=["f64", "f32"], default="f32")
+    ap.add_argument("--reps", type=int, default=50)
+    ap.add_argument("--missing-frac", type=float, default=0.05)
+    ap.add_argument("--copy-mib", type=int, default=0)
+    args = ap.parse_args()
+
+    import numpy as np
+    import torch
+
+    dev = torch.device("cuda:0")
+    if args.copy_mib:
+        n = args.copy_mib * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device=dev)
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": 4 * n}), flush=True)
+        return 0
+
+    from csipm import _lib
+    from csipm.fft_operator import FFTOperator, _ptr
+
+    dims = tuple(args.dims)
+    N = int(np.prod(dims))
+    rng = np.random.default_rng(0)
+    miss = np.sort(rng.choice(N, size=int(args.missing_frac * N), replace=False))
+    dt = torch.float64 if args.dtype == "f64" else torch.float32
+    op = FFTOperator(N, len(dims), dims, miss, dtype=dt)
+    x = torch.randn(2 * N, dtype=dt, device=dev)
+    L1 = torch.rand(N, dtype=dt, device=dev) + 0.1
+    L2 = torch.rand(N, dtype=dt, device=dev) - 0.5
+    y = torch.empty(2 * N, dtype=dt, device=dev)
+    ms = (ctypes.c_double * 8)()
+    by = (ctypes.c_double * 8)()
+    npass = ctypes.c_int()
+    h = op.handle
+    _lib.call("csipm_kkt_apply_timed", h, _ptr(x), _ptr(y), _ptr(L1), _ptr(L2), 3, ms, by,
+              ctypes.byref(npass))
+    _lib.call("csipm_kkt_apply_timed", h, _ptr(x), _ptr(y), _ptr(L1), _ptr(L2), args.reps, ms,
+              by, ctypes.byref(npass))
+    passes = {}
+    tot_ms = 0.0
+    tot_b = 0.0
+    for i in range(npass.value):
+        name = _lib.lib().csipm_kkt_pass_name(h, i).decode()
+        gbs = by[i] / (ms[i] * 1e-3) / 1e9
+        passes[name] = {"ms": round(ms[i], 5), "bytes": int(by[i]), "GB/s": round(gbs, 1),
+                        "frac": round(gbs / HBM_PEAK_GBS, 4)}
+        tot_ms += ms[i]
+        tot_b += by[i]
+    s = 8 if args.dtype == "f64" else 4
+    nc = (dims[0] // 2 + 1) * (N // dims[0])
+    R = 0 if len(dims) == 1 else 2 * (len(dims) - 1)
+    bk = 6 * N * s + (N + 7) // 8 + R * 4 * nc * s
+    out = {"dims": list(dims), "dtype": args.dtype, "N": N, "reps": args.reps,
+           "k_apply_ms": round(tot_ms, 5), "B_K": bk,
+           "k_apply_GB/s": round(bk / (tot_ms * 1e-3) / 1e9, 1),
+           "k_apply_frac": round(bk / (tot_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+           "passes": passes}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
