@@ -115,21 +115,23 @@ def main():
     total = time.perf_counter() - t_all
 
     # roofline of the dominant pass of the fused K apply
-    n = nlp.nbeta
+    # the K apply the CG runs (spectrum-ordered vectors of length U = 2*Nc)
+    n = 2 * nlp.op_fft.Nc
+    dt = nlp.op_fft.dtype
     rng = np.random.default_rng(0)
-    x = torch.from_numpy(rng.standard_normal(2 * n)).to(dev)
-    L1 = torch.from_numpy(rng.uniform(0.1, 3, n)).to(dev)
-    L2 = torch.from_numpy(rng.uniform(-1, 1, n)).to(dev)
-    y = torch.empty(2 * n, dtype=torch.float64, device=dev)
+    x = torch.from_numpy(rng.standard_normal(2 * n)).to(dev, dt)
+    L1 = torch.from_numpy(rng.uniform(0.1, 3, n)).to(dev, dt)
+    L2 = torch.from_numpy(rng.uniform(-1, 1, n)).to(dev, dt)
+    y = torch.empty(2 * n, dtype=dt, device=dev)
     ms = (ctypes.c_double * 8)()
     by = (ctypes.c_double * 8)()
     npass = ctypes.c_int()
     h = nlp.op_fft.handle
-    _lib.call("csipm_kkt_apply_timed", h, _ptr(x), _ptr(y), _ptr(L1), _ptr(L2), 20, ms, by,
+    _lib.call("csipm_kkt_apply_timed", h, _ptr(x), _ptr(y), _ptr(L1), _ptr(L2), 20, 1, ms, by,
               ctypes.byref(npass))
-    _lib.call("csipm_kkt_apply_timed", h, _ptr(x), _ptr(y), _ptr(L1), _ptr(L2), 200, ms, by,
+    _lib.call("csipm_kkt_apply_timed", h, _ptr(x), _ptr(y), _ptr(L1), _ptr(L2), 200, 1, ms, by,
               ctypes.byref(npass))
-    passes = [(_lib.lib().csipm_kkt_pass_name(h, i).decode(), ms[i], by[i])
+    passes = [(_lib.lib().csipm_kkt_pass_name(h, 1, i).decode(), ms[i], by[i])
               for i in range(npass.value)]
     name, pms, pbytes = max(passes, key=lambda p: p[1])
     achieved = pbytes / (pms * 1e-3) / 1e9

@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(_HERE, "libcsipm.so")
 
 OK = 0
 ERR_ARG, ERR_SHAPE, ERR_ALLOC, ERR_HIP, ERR_UNSUPPORTED, ERR_NOT_CONVERGED = 1, 2, 3, 4, 5, 6
-F64, F32 = 0, 1
+F64, F32, MIXED = 0, 1, 2
 
 # every exported symbol, with (restype, argtypes)
 _vp = ctypes.c_void_p
@@ -47,9 +47,9 @@ SIGNATURES = {
     "csipm_jtprod": (_i, [_vp, _vp, _vp]),
     "csipm_condense_rhs": (_i, [_vp, _vp, _vp, _vp, _vp]),
     "csipm_unpack_solution": (_i, [_vp, _vp, _vp, _vp, _vp]),
-    "csipm_kkt_apply_timed": (_i, [_vp, _vp, _vp, _vp, _vp, _i64, ctypes.POINTER(_d),
+    "csipm_kkt_apply_timed": (_i, [_vp, _vp, _vp, _vp, _vp, _i64, _i, ctypes.POINTER(_d),
                                    ctypes.POINTER(_d), ctypes.POINTER(_i)]),
-    "csipm_kkt_pass_name": (ctypes.c_char_p, [_vp, _i]),
+    "csipm_kkt_pass_name": (ctypes.c_char_p, [_vp, _i, _i]),
 }
 
 

@@ -50,10 +50,14 @@ class FFTParameters:
 
 
 class FFTOperator:
-    """Device operator handle: twiddles, mask, scratch spectra, CG scalars."""
+    """Device operator handle: twiddles, mask, scratch spectra, CG scalars.
+
+    dtype float64 / float32 selects the operator arithmetic; mixed=True makes
+    a handle whose operator and CG run in FP32 inside the FP64 interior-point
+    driver (its operator entry points take float32 vectors)."""
 
     def __init__(self, nbeta, DFTdim, DFTsize, index_missing, rdft=True, dtype=torch.float64,
-                 device="cuda"):
+                 device="cuda", mixed=False):
         if not rdft:
             raise NotImplementedError("only the rdft=true (R2C/C2R) path is implemented")
         self.DFTdim = int(DFTdim)
@@ -62,11 +66,14 @@ class FFTOperator:
         if int(nbeta) != self.N:
             raise ValueError("nbeta must equal prod(DFTsize)")
         self.rdft = True
-        self.dtype = dtype
-        self.device = torch.device(device)
-        prec = _lib.F64 if dtype == torch.float64 else _lib.F32
+        if mixed:
+            dtype = torch.float32     # the operator entry points run in FP32
         if dtype not in (torch.float64, torch.float32):
             raise ValueError("dtype must be float64 or float32")
+        self.dtype = dtype
+        self.mixed = bool(mixed)
+        self.device = torch.device(device)
+        prec = _lib.MIXED if mixed else (_lib.F64 if dtype == torch.float64 else _lib.F32)
         sizes = (ctypes.c_int64 * 3)(*(list(self.DFTsize) + [1] * (3 - len(self.DFTsize))))
         miss = np.ascontiguousarray(np.asarray(index_missing, dtype=np.int64).reshape(-1))
         self.index_missing = miss

@@ -28,7 +28,7 @@ class FFTNLPModel:
     beta - c <= 0 (fft_model.jl:20-34)."""
 
     def __init__(self, parameters: FFTParameters, krylov_solver="cg", rdft=True,
-                 preconditioner=True, dtype=torch.float64, device="cuda"):
+                 preconditioner=True, dtype=torch.float64, device="cuda", mixed=False):
         if krylov_solver != "cg":
             raise NotImplementedError("only krylov_solver='cg' is implemented")
         self.parameters = parameters
@@ -38,7 +38,9 @@ class FFTNLPModel:
         self.krylov_solver = krylov_solver
         self.preconditioner = preconditioner
         self.op_fft = FFTOperator(self.nbeta, parameters.DFTdim, parameters.DFTsize,
-                                  parameters.index_missing, rdft, dtype=dtype, device=device)
+                                  parameters.index_missing, rdft, dtype=dtype, device=device,
+                                  mixed=mixed)
+        dtype = self.op_fft.dtype
         self.M_perpt_z0 = M_perpt_z(self.op_fft, parameters.z0)
         self.counters = {"neval_obj": 0, "neval_grad": 0, "neval_cons": 0}
         dev = self.op_fft.device
@@ -220,10 +222,12 @@ class ExecutionStats:
 def ipm_solve(nlp: FFTNLPModel, tol=1e-8, max_iter=200, cg_rtol=1e-10, cg_itmax=0):
     """Solve the model with the library's interior-point driver
     (csipm_solve; the reference runs MadNLP, src/CompressedSensingIPM.jl:31-36).
-    Returns the primal beta (device) and the driver's statistics."""
+    Returns the primal beta (device, FP64) and the driver's statistics.  On a
+    mixed-precision model the CG tolerance is floored at 1e-5 (FP32 CG) and
+    every Newton direction gets one FP64 refinement step."""
     op = nlp.op_fft
-    if op.dtype != torch.float64:
-        raise NotImplementedError("the driver runs on an FP64 operator")
+    if op.dtype != torch.float64 and not op.mixed:
+        raise NotImplementedError("the driver runs on an FP64 or a mixed-precision operator")
     z0 = op._vec(nlp.parameters.z0, dtype=torch.float64)
     beta = torch.empty(op.N, dtype=torch.float64, device=op.device)
     info = (ctypes.c_double * 8)()

@@ -27,6 +27,11 @@ def main():
     ap.add_argument("--reps", type=int, default=50)
     ap.add_argument("--missing-frac", type=float, default=0.05)
     ap.add_argument("--copy-mib", type=int, default=0)
+    ap.add_argument("--layout", choices=["spectral", "beta", "reduced"], default="spectral",
+                    help="spectral: the condensed-K apply inside CG (spectrum-ordered "
+                         "vectors); beta: csipm_kkt_apply on reference-ordered vectors; "
+                         "reduced: the (M^T M + D) apply of the mixed-precision CG")
+    ap.add_argument("--mixed", action="store_true", help="FP32 operator of a mixed handle")
     args = ap.parse_args()
 
     import numpy as np
@@ -50,37 +55,38 @@ def main():
     rng = np.random.default_rng(0)
     miss = np.sort(rng.choice(N, size=int(args.missing_frac * N), replace=False))
     dt = torch.float64 if args.dtype == "f64" else torch.float32
-    op = FFTOperator(N, len(dims), dims, miss, dtype=dt)
-    x = torch.randn(2 * N, dtype=dt, device=dev)
-    L1 = torch.rand(N, dtype=dt, device=dev) + 0.1
-    L2 = torch.rand(N, dtype=dt, device=dev) - 0.5
-    y = torch.empty(2 * N, dtype=dt, device=dev)
+    op = FFTOperator(N, len(dims), dims, miss, dtype=dt, mixed=args.mixed)
+    dt = op.dtype
+    layout = {"beta": 0, "spectral": 1, "reduced": 2}[args.layout]
+    n = N if layout == 0 else 2 * op.Nc
+    x = torch.randn(2 * n, dtype=dt, device=dev)
+    L1 = torch.rand(n, dtype=dt, device=dev) + 0.1
+    L2 = torch.rand(n, dtype=dt, device=dev) - 0.5
+    y = torch.empty(2 * n, dtype=dt, device=dev)
     ms = (ctypes.c_double * 8)()
     by = (ctypes.c_double * 8)()
     npass = ctypes.c_int()
     h = op.handle
-    _lib.call("csipm_kkt_apply_timed", h, _ptr(x), _ptr(y), _ptr(L1), _ptr(L2), 3, ms, by,
-              ctypes.byref(npass))
-    _lib.call("csipm_kkt_apply_timed", h, _ptr(x), _ptr(y), _ptr(L1), _ptr(L2), args.reps, ms,
+    _lib.call("csipm_kkt_apply_timed", h, _ptr(x), _ptr(y), _ptr(L1), _ptr(L2), 3, layout, ms,
               by, ctypes.byref(npass))
+    _lib.call("csipm_kkt_apply_timed", h, _ptr(x), _ptr(y), _ptr(L1), _ptr(L2), args.reps,
+              layout, ms, by, ctypes.byref(npass))
     passes = {}
     tot_ms = 0.0
     tot_b = 0.0
     for i in range(npass.value):
-        name = _lib.lib().csipm_kkt_pass_name(h, i).decode()
+        name = _lib.lib().csipm_kkt_pass_name(h, layout, i).decode()
         gbs = by[i] / (ms[i] * 1e-3) / 1e9
         passes[name] = {"ms": round(ms[i], 5), "bytes": int(by[i]), "GB/s": round(gbs, 1),
                         "frac": round(gbs / HBM_PEAK_GBS, 4)}
         tot_ms += ms[i]
         tot_b += by[i]
-    s = 8 if args.dtype == "f64" else 4
-    nc = (dims[0] // 2 + 1) * (N // dims[0])
-    R = 0 if len(dims) == 1 else 2 * (len(dims) - 1)
-    bk = 6 * N * s + (N + 7) // 8 + R * 4 * nc * s
-    out = {"dims": list(dims), "dtype": args.dtype, "N": N, "reps": args.reps,
-           "k_apply_ms": round(tot_ms, 5), "B_K": bk,
-           "k_apply_GB/s": round(bk / (tot_ms * 1e-3) / 1e9, 1),
-           "k_apply_frac": round(bk / (tot_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+    s = 8 if dt == torch.float64 else 4
+    out = {"dims": list(dims), "dtype": "f64" if s == 8 else "f32", "layout": args.layout,
+           "N": N, "U": 2 * op.Nc, "reps": args.reps,
+           "k_apply_ms": round(tot_ms, 5), "bytes": int(tot_b),
+           "k_apply_GB/s": round(tot_b / (tot_ms * 1e-3) / 1e9, 1),
+           "k_apply_frac": round(tot_b / (tot_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
            "passes": passes}
     print(json.dumps(out), flush=True)
     return 0

@@ -354,11 +354,15 @@ __device__ __forceinline__ void dft(cpx<T>* v) {
 }
 
 // ---------------------------------------------------------------------------
-// Stockham autosort FFT of `nlines` lines of length L held in LDS (line stride
-// ls).  Stage with radix R after Ns = product of earlier radices: butterfly j
-// reads elements j + r*L/R, twiddles by w^(r*(j mod Ns)*L/(Ns*R)), and writes
-// (j - j mod Ns)*R + j mod Ns + r*Ns.  Each thread stages at most E elements
-// in registers; the caller guarantees nlines*L <= blockDim.x*E.
+// Stockham autosort FFT of 2^lg lines of length L held in LDS in
+// [element][line] order: element e of line l at lds[e * ps + l] (ps >= 2^lg,
+// the line stride, padded where the load loops would otherwise conflict).
+// Stage s with radix R after Ns = product of earlier radices: butterfly j
+// reads elements j + r*L/R, multiplies input r by w^(r*(j mod Ns)*L/(Ns*R)),
+// and writes elements (j - j mod Ns)*R + j mod Ns + r*Ns.  Butterflies are
+// numbered b = j * 2^lg + l, so the lines of a tile run on adjacent lanes and
+// share twiddles.  Each thread stages at most E elements per stage in
+// registers; the caller guarantees 2^lg * L <= NT * E.
 // ---------------------------------------------------------------------------
 constexpr int kMaxStages = 20;
 
@@ -366,38 +370,46 @@ struct FftPlan {
   int L;
   int nstages;
   int radix[kMaxStages];
+  int ns[kMaxStages];        // product of the radices of earlier stages
+  int tws[kMaxStages];       // twiddle index stride L / (ns * radix)
+  uint32_t nsm[kMaxStages];  // ceil(2^32 / ns), ns >= 2: j / ns = umulhi(j, nsm)
 };
 
-template <typename T, int R, int E, bool FWD>
-__device__ __forceinline__ void stockham_stage(cpx<T>* __restrict__ lds, int L, int Ns,
-                                               int nlines, int ls,
-                                               const cpx<T>* __restrict__ tw) {
+// exact for a, d <= 2^16 (host checks the plan limits)
+__device__ __forceinline__ int fdiv(int a, uint32_t m) { return (int)__umulhi((uint32_t)a, m); }
+
+template <typename T, int R, int E, int NT, bool FWD>
+__device__ __forceinline__ void stockham_stage(cpx<T>* __restrict__ lds, const FftPlan& p, int s,
+                                               int lg, int ps, const cpx<T>* __restrict__ tw) {
   constexpr int NQ = (E + R - 1) / R;
+  const int L = p.L;
   const int nb = L / R;
-  const int total = nb * nlines;
-  const int nt = blockDim.x;
+  const int nl = 1 << lg;
+  const int total = nb << lg;
+  const int Ns = p.ns[s];
   cpx<T> v[NQ][R];
 #pragma unroll
   for (int q = 0; q < NQ; ++q) {
-    const int b = threadIdx.x + q * nt;
+    const int b = threadIdx.x + q * NT;
     if (b < total) {
-      const int line = b / nb;
-      const int j = b - line * nb;
-      const cpx<T>* src = lds + line * ls + j;
+      const int j = b >> lg;
+      const int line = b & (nl - 1);
+      const cpx<T>* src = lds + j * ps + line;
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[q][r] = src[r * nb];
+      for (int r = 0; r < R; ++r) v[q][r] = src[r * nb * ps];
     }
   }
   __syncthreads();
 #pragma unroll
   for (int q = 0; q < NQ; ++q) {
-    const int b = threadIdx.x + q * nt;
+    const int b = threadIdx.x + q * NT;
     if (b < total) {
-      const int line = b / nb;
-      const int j = b - line * nb;
-      const int k = j % Ns;
+      const int j = b >> lg;
+      const int line = b & (nl - 1);
+      int k = 0;
       if (Ns > 1) {
-        const int step = k * (L / (Ns * R));
+        k = j - fdiv(j, p.nsm[s]) * Ns;
+        const int step = k * p.tws[s];
 #pragma unroll
         for (int r = 1; r < R; ++r) {
           cpx<T> w = tw[r * step];
@@ -406,37 +418,82 @@ __device__ __forceinline__ void stockham_stage(cpx<T>* __restrict__ lds, int L, 
         }
       }
       dft<T, R, FWD>(v[q]);
-      cpx<T>* dst = lds + line * ls + (j - k) * R + k;
+      cpx<T>* dst = lds + ((j - k) * R + k) * ps + line;
 #pragma unroll
-      for (int r = 0; r < R; ++r) dst[r * Ns] = v[q][r];
+      for (int r = 0; r < R; ++r) dst[r * Ns * ps] = v[q][r];
     }
   }
   __syncthreads();
 }
 
-template <typename T, int E, bool FWD>
-__device__ void lds_fft(cpx<T>* lds, int nlines, int ls, const FftPlan& plan,
+template <typename T, int E, int NT, bool FWD>
+__device__ void lds_fft(cpx<T>* lds, const FftPlan& plan, int lg, int ps,
                         const cpx<T>* __restrict__ tw) {
-  int Ns = 1;
-  const int L = plan.L;
   for (int s = 0; s < plan.nstages; ++s) {
-    const int R = plan.radix[s];
-    switch (R) {
-      case 2: stockham_stage<T, 2, E, FWD>(lds, L, Ns, nlines, ls, tw); break;
-      case 3: stockham_stage<T, 3, E, FWD>(lds, L, Ns, nlines, ls, tw); break;
-      case 4: stockham_stage<T, 4, E, FWD>(lds, L, Ns, nlines, ls, tw); break;
-      case 5: stockham_stage<T, 5, E, FWD>(lds, L, Ns, nlines, ls, tw); break;
-      case 7: stockham_stage<T, 7, E, FWD>(lds, L, Ns, nlines, ls, tw); break;
-      case 8: stockham_stage<T, 8, E, FWD>(lds, L, Ns, nlines, ls, tw); break;
-      case 9: stockham_stage<T, 9, E, FWD>(lds, L, Ns, nlines, ls, tw); break;
-      case 11: stockham_stage<T, 11, E, FWD>(lds, L, Ns, nlines, ls, tw); break;
-      case 13: stockham_stage<T, 13, E, FWD>(lds, L, Ns, nlines, ls, tw); break;
-      case 16: stockham_stage<T, 16, E, FWD>(lds, L, Ns, nlines, ls, tw); break;
-      case 17: stockham_stage<T, 17, E, FWD>(lds, L, Ns, nlines, ls, tw); break;
+    switch (plan.radix[s]) {
+      case 2: stockham_stage<T, 2, E, NT, FWD>(lds, plan, s, lg, ps, tw); break;
+      case 3: stockham_stage<T, 3, E, NT, FWD>(lds, plan, s, lg, ps, tw); break;
+      case 4: stockham_stage<T, 4, E, NT, FWD>(lds, plan, s, lg, ps, tw); break;
+      case 5: stockham_stage<T, 5, E, NT, FWD>(lds, plan, s, lg, ps, tw); break;
+      case 7: stockham_stage<T, 7, E, NT, FWD>(lds, plan, s, lg, ps, tw); break;
+      case 8: stockham_stage<T, 8, E, NT, FWD>(lds, plan, s, lg, ps, tw); break;
+      case 9: stockham_stage<T, 9, E, NT, FWD>(lds, plan, s, lg, ps, tw); break;
+      case 11: stockham_stage<T, 11, E, NT, FWD>(lds, plan, s, lg, ps, tw); break;
+      case 13: stockham_stage<T, 13, E, NT, FWD>(lds, plan, s, lg, ps, tw); break;
+      case 15: stockham_stage<T, 15, E, NT, FWD>(lds, plan, s, lg, ps, tw); break;
+      case 16: stockham_stage<T, 16, E, NT, FWD>(lds, plan, s, lg, ps, tw); break;
+      case 17: stockham_stage<T, 17, E, NT, FWD>(lds, plan, s, lg, ps, tw); break;
       default: break;  // the host planner only emits the radices above
     }
-    Ns *= R;
   }
+}
+
+// ---------------------------------------------------------------------------
+// Spectrum-ordered copy of beta ("u"): one complex slot per half-spectrum
+// element e = k1 + H1*(k2 + N2*k3); stored elements hold
+// u = sqrt(2) * v (v.x for vertices, whose Im slot is 0), redundant
+// conjugates in the self-conjugate dim-1 planes hold 0.  Relative to beta it
+// is a permutation with sign flips (the 3D mirror blocks' Im parts), so dot
+// products, the diagonal KKT terms and the CG iterates carry over exactly,
+// while the FFT passes read and write it along contiguous lines.
+// ---------------------------------------------------------------------------
+template <int D, typename T>
+__host__ __device__ __forceinline__ cpx<T> u_to_v(const cpx<T>* __restrict__ u, int k1, int k2,
+                                                  int k3, const MapGeom& g, int H1) {
+#pragma clang fp contract(off)
+  const T sq2 = (T)1.4142135623730951;
+  const Cat c1 = cat_of(k1, g, 0);
+  const int64_t e = k1 + (int64_t)H1 * (k2 + (int64_t)g.n[1] * k3);
+  if (c1.c == 1) {
+    const cpx<T> w = u[e];
+    return {w.x / sq2, w.y / sq2};
+  }
+  const Cat c2 = D >= 2 ? cat_of(k2, g, 1) : Cat{0, 0};
+  const Cat c3 = D >= 3 ? cat_of(k3, g, 2) : Cat{0, 0};
+  if (is_rep<D>(c1, c2, c3)) {
+    const cpx<T> w = u[e];
+    if (c2.c == 0 && c3.c == 0) return {w.x, (T)0};
+    return {w.x / sq2, w.y / sq2};
+  }
+  const int m2 = k2 == 0 ? 0 : g.n[1] - k2;
+  const int m3 = k3 == 0 ? 0 : g.n[2] - k3;
+  const cpx<T> w = u[k1 + (int64_t)H1 * (m2 + (int64_t)g.n[1] * m3)];
+  return {w.x / sq2, -(w.y / sq2)};
+}
+
+// u slot value of spectrum element (k1,k2,k3) holding v
+template <int D, typename T>
+__host__ __device__ __forceinline__ cpx<T> v_to_u(cpx<T> v, int k1, int k2, int k3,
+                                                  const MapGeom& g) {
+#pragma clang fp contract(off)
+  const T sq2 = (T)1.4142135623730951;
+  const Cat c1 = cat_of(k1, g, 0);
+  if (c1.c == 1) return {sq2 * v.x, sq2 * v.y};
+  const Cat c2 = D >= 2 ? cat_of(k2, g, 1) : Cat{0, 0};
+  const Cat c3 = D >= 3 ? cat_of(k3, g, 2) : Cat{0, 0};
+  if (!is_rep<D>(c1, c2, c3)) return {(T)0, (T)0};
+  if (c2.c == 0 && c3.c == 0) return {v.x, (T)0};
+  return {sq2 * v.x, sq2 * v.y};
 }
 
 // ---------------------------------------------------------------------------

@@ -173,11 +173,12 @@ __global__ __launch_bounds__(256) void k_dft_to_beta(const cpx<double>* __restri
 }
 
 // ===========================================================================
-// Destinations of the forward passes that end in beta space
+// Destinations of the forward passes that end in beta (or u) space
 // ===========================================================================
 template <typename T>
 struct Epi {
   int kind;             // 0: out[i] = value          1: condensed KKT epilogue
+                        // 2: reduced (Schur) epilogue y = alpha (m + D x) + beta y, D in L1
   T* out;
   const T* xb;
   const T* xz;
@@ -196,6 +197,13 @@ __device__ __forceinline__ double epi_apply(const Epi<T>& e, int64_t i, T m) {
     e.out[i] = m;
     return 0.0;
   }
+  if (e.kind == 2) {
+    const T xb = e.xb[i];
+    T qb = e.alpha * (m + e.L1[i] * xb);
+    if (e.beta != (T)0) qb += e.beta * e.yb[i];
+    e.yb[i] = qb;
+    return (double)xb * (double)qb;
+  }
   const T xb = e.xb[i], xz = e.xz[i], l1 = e.L1[i], l2 = e.L2[i];
   T qb = e.alpha * (m + l1 * xb + l2 * xz);
   T qz = e.alpha * (l2 * xb + l1 * xz);
@@ -206,6 +214,47 @@ __device__ __forceinline__ double epi_apply(const Epi<T>& e, int64_t i, T m) {
   e.yb[i] = qb;
   e.yz[i] = qz;
   return (double)xb * (double)qb + (double)xz * (double)qz;
+}
+
+// the same on the two slots (re, im) of u-space element e, as one vector access
+template <typename T>
+__device__ __forceinline__ double epi_apply_u(const Epi<T>& e, int64_t el, cpx<T> m) {
+  if (e.kind == 0) {
+    reinterpret_cast<cpx<T>*>(e.out)[el] = m;
+    return 0.0;
+  }
+  if (e.kind == 2) {
+    const cpx<T> xb = reinterpret_cast<const cpx<T>*>(e.xb)[el];
+    const cpx<T> dg = reinterpret_cast<const cpx<T>*>(e.L1)[el];
+    cpx<T> qb = {e.alpha * (m.x + dg.x * xb.x), e.alpha * (m.y + dg.y * xb.y)};
+    cpx<T>* yb = reinterpret_cast<cpx<T>*>(e.yb);
+    if (e.beta != (T)0) {
+      const cpx<T> ob = yb[el];
+      qb.x += e.beta * ob.x;
+      qb.y += e.beta * ob.y;
+    }
+    yb[el] = qb;
+    return (double)xb.x * qb.x + (double)xb.y * qb.y;
+  }
+  const cpx<T> xb = reinterpret_cast<const cpx<T>*>(e.xb)[el];
+  const cpx<T> xz = reinterpret_cast<const cpx<T>*>(e.xz)[el];
+  const cpx<T> l1 = reinterpret_cast<const cpx<T>*>(e.L1)[el];
+  const cpx<T> l2 = reinterpret_cast<const cpx<T>*>(e.L2)[el];
+  cpx<T> qb = {e.alpha * (m.x + l1.x * xb.x + l2.x * xz.x),
+               e.alpha * (m.y + l1.y * xb.y + l2.y * xz.y)};
+  cpx<T> qz = {e.alpha * (l2.x * xb.x + l1.x * xz.x), e.alpha * (l2.y * xb.y + l1.y * xz.y)};
+  cpx<T>* yb = reinterpret_cast<cpx<T>*>(e.yb);
+  cpx<T>* yz = reinterpret_cast<cpx<T>*>(e.yz);
+  if (e.beta != (T)0) {
+    const cpx<T> ob = yb[el], oz = yz[el];
+    qb.x += e.beta * ob.x;
+    qb.y += e.beta * ob.y;
+    qz.x += e.beta * oz.x;
+    qz.y += e.beta * oz.y;
+  }
+  yb[el] = qb;
+  yz[el] = qz;
+  return (double)xb.x * qb.x + (double)xb.y * qb.y + (double)xz.x * qz.x + (double)xz.y * qz.y;
 }
 
 // value v of half-spectrum element (k1,k2,k3) -> the beta entries reading it
@@ -229,105 +278,107 @@ __device__ __forceinline__ void store_partial(double* part, double acc) {
 }
 
 // ===========================================================================
-// Line FFTs along dimension 2 or 3 of the (H1, N2, N3) half spectrum.
-// A workgroup takes nl lines with consecutive k1 (coalesced strided access).
+// FFT passes over the (H1, N2, N3) half spectrum.
+//  k_lines: lines along dimension 2 (axis 1) or 3 (axis 2); a workgroup takes
+//           2^lg lines with consecutive k1, so global accesses are contiguous.
+//  k_dim1:  dimension 1; two real lines per complex FFT of length N1
+//           (z = x_a + i x_b), 2^lg pairs per workgroup.
+// Sources: SRC_BUF (scratch spectrum), SRC_BETA (gather from beta through the
+// packing map), SRC_U (spectrum-ordered beta), SRC_REAL (real samples, dim 1).
+// Destinations likewise, the beta / u ones with the epilogue Epi.
 // ===========================================================================
-enum { SRC_BUF = 0, SRC_BETA = 1, SRC_REAL = 2 };
-enum { DST_BUF = 0, DST_BETA = 1, DST_REAL = 2 };
+enum { SRC_BUF = 0, SRC_BETA = 1, SRC_REAL = 2, SRC_U = 3 };
+enum { DST_BUF = 0, DST_BETA = 1, DST_REAL = 2, DST_U = 3 };
+enum { MODE_MTM = 0, MODE_C2R = 1, MODE_R2C = 2 };
 
 template <typename T>
-struct LineArgs {
+struct PassArgs {
   MapGeom g;
   FftPlan plan;
   const cpx<T>* tw;
-  int axis;     // 1: lines along dim 2; 2: lines along dim 3
+  int axis;      // k_lines: 1 or 2
   int H1;
-  int nl;
-  int ntile1;
+  int lg;        // log2 of lines (pairs) per workgroup
+  int ps;        // LDS line stride
+  int ntile1;    // k_lines: tiles along k1
+  int nlines;    // k_dim1: real lines N2*N3
+  int npairs;
+  uint32_t h1m;  // ceil(2^32 / H1)
+  uint32_t n1m;  // ceil(2^32 / N1)
   cpx<T>* buf;
   const T* beta_in;
+  const cpx<T>* u_in;
+  const T* real_in;
+  T* real_out;
   Epi<T> epi;
+  const uint32_t* miss;  // bit set = missing sample (nullable)
+  T scale;
   const int* done;
 };
 
+__device__ __forceinline__ int udiv(int a, int d, uint32_t m) { return d == 1 ? a : fdiv(a, m); }
+
+template <int D, typename T, int SRC>
+__device__ __forceinline__ cpx<T> spec_load(const PassArgs<T>& a, int k1, int k2, int k3) {
+  if (SRC == SRC_BETA) return beta_to_dft_elem<D, T>(a.beta_in, k1, k2, k3, a.g);
+  return u_to_v<D, T>(a.u_in, k1, k2, k3, a.g, a.H1);
+}
+
+template <int D, typename T, int DST>
+__device__ __forceinline__ double spec_store(const PassArgs<T>& a, int k1, int k2, int k3,
+                                             cpx<T> v) {
+  if (DST == DST_BETA) return scatter_elem<D, T>(a.epi, k1, k2, k3, v, a.g);
+  const int64_t el = k1 + (int64_t)a.H1 * (k2 + (int64_t)a.g.n[1] * k3);
+  return epi_apply_u(a.epi, el, v_to_u<D, T>(v, k1, k2, k3, a.g));
+}
+
 template <typename T, int NT, int E, int D, int SRC, int DST, bool FWD>
-__global__ __launch_bounds__(NT) void k_lines(LineArgs<T> a) {
+__global__ __launch_bounds__(NT) void k_lines(PassArgs<T> a) {
   extern __shared__ unsigned char smem_raw[];
   cpx<T>* lds = reinterpret_cast<cpx<T>*>(smem_raw);
   if (a.done && *a.done) return;
-  const MapGeom& g = a.g;
   const int L = a.plan.L;
+  const int lg = a.lg, nl = 1 << lg;
   const int tile1 = blockIdx.x % a.ntile1;
   const int o = blockIdx.x / a.ntile1;
-  const int k10 = tile1 * a.nl;
-  const int nl = min(a.nl, a.H1 - k10);
-  const int n2 = g.n[1];
-  const int64_t s2 = a.H1, s3 = (int64_t)a.H1 * n2;
-  const int64_t sl = a.axis == 1 ? s2 : s3;                  // stride along the line
+  const int k10 = tile1 << lg;
+  const int nv = min(nl, a.H1 - k10);
+  const int64_t s2 = a.H1, s3 = (int64_t)a.H1 * a.g.n[1];
+  const int64_t sl = a.axis == 1 ? s2 : s3;
   const int64_t base = k10 + (a.axis == 1 ? (int64_t)o * s3 : (int64_t)o * s2);
-  const int total = nl * L;
+  const int total = L << lg;
   for (int idx = threadIdx.x; idx < total; idx += NT) {
-    const int line = idx % nl;
-    const int el = idx / nl;
-    cpx<T> val;
-    if (SRC == SRC_BUF) {
-      val = a.buf[base + line + el * sl];
-    } else {
-      const int k2 = a.axis == 1 ? el : o;
-      const int k3 = a.axis == 1 ? o : el;
-      val = beta_to_dft_elem<D, T>(a.beta_in, k10 + line, k2, k3, g);
+    const int line = idx & (nl - 1);
+    const int el = idx >> lg;
+    cpx<T> val = {(T)0, (T)0};
+    if (line < nv) {
+      if (SRC == SRC_BUF) {
+        val = a.buf[base + line + el * sl];
+      } else {
+        const int k2 = a.axis == 1 ? el : o;
+        const int k3 = a.axis == 1 ? o : el;
+        val = spec_load<D, T, SRC>(a, k10 + line, k2, k3);
+      }
     }
-    lds[line * L + el] = val;
+    lds[idx] = val;
   }
   __syncthreads();
-  lds_fft<T, E, FWD>(lds, nl, L, a.plan, a.tw);
+  lds_fft<T, E, NT, FWD>(lds, a.plan, lg, nl, a.tw);
   double acc = 0.0;
   for (int idx = threadIdx.x; idx < total; idx += NT) {
-    const int line = idx % nl;
-    const int el = idx / nl;
-    const cpx<T> val = lds[line * L + el];
+    const int line = idx & (nl - 1);
+    const int el = idx >> lg;
+    if (line >= nv) continue;
+    const cpx<T> val = lds[idx];
     if (DST == DST_BUF) {
       a.buf[base + line + el * sl] = val;
     } else {
       const int k2 = a.axis == 1 ? el : o;
       const int k3 = a.axis == 1 ? o : el;
-      acc += scatter_elem<D, T>(a.epi, k10 + line, k2, k3, val, g);
+      acc += spec_store<D, T, DST>(a, k10 + line, k2, k3, val);
     }
   }
-  if (DST == DST_BETA && a.epi.dot_part) store_partial(a.epi.dot_part, acc);
-}
-
-// ===========================================================================
-// Dimension-1 pass: two real lines per complex FFT of length N1.
-// MODE 0 (MtM): half spectra -> C2R -> mask*scale -> R2C -> half spectra
-// MODE 1 (C2R): half spectra -> C2R -> mask*scale -> real samples
-// MODE 2 (R2C): real samples*scale -> R2C -> half spectra
-// ===========================================================================
-enum { MODE_MTM = 0, MODE_C2R = 1, MODE_R2C = 2 };
-
-template <typename T>
-struct Dim1Args {
-  MapGeom g;
-  FftPlan plan;
-  const cpx<T>* tw;
-  int N1, H1;
-  int nlines;     // real lines = N2*N3
-  int npairs;
-  int np;         // pairs per workgroup
-  cpx<T>* buf;
-  const T* beta_in;
-  const T* real_in;
-  T* real_out;
-  Epi<T> epi;
-  const uint32_t* miss;   // bit set = missing sample (nullable)
-  T scale;
-  const int* done;
-};
-
-template <typename T, int D, int SRC>
-__device__ __forceinline__ cpx<T> d1_load(const Dim1Args<T>& a, int line, int k) {
-  if (SRC == SRC_BUF) return a.buf[(int64_t)line * a.H1 + k];
-  return beta_to_dft_elem<D, T>(a.beta_in, k, 0, 0, a.g);   // 1D: line 0
+  if (DST != DST_BUF && a.epi.dot_part) store_partial(a.epi.dot_part, acc);
 }
 
 __device__ __forceinline__ bool miss_bit(const uint32_t* m, int64_t s) {
@@ -335,84 +386,156 @@ __device__ __forceinline__ bool miss_bit(const uint32_t* m, int64_t s) {
 }
 
 template <typename T, int NT, int E, int D, int SRC, int DST, int MODE>
-__global__ __launch_bounds__(NT) void k_dim1(Dim1Args<T> a) {
+__global__ __launch_bounds__(NT) void k_dim1(PassArgs<T> a) {
   extern __shared__ unsigned char smem_raw[];
   cpx<T>* lds = reinterpret_cast<cpx<T>*>(smem_raw);
   if (a.done && *a.done) return;
-  const int N1 = a.N1, H1 = a.H1;
-  const int p0 = blockIdx.x * a.np;
-  const int npl = min(a.np, a.npairs - p0);
+  const int N1 = a.plan.L, H1 = a.H1;
+  const int lg = a.lg, np = 1 << lg, ps = a.ps;
+  const int p0 = blockIdx.x << lg;
   const bool even = (N1 & 1) == 0;
   if (MODE != MODE_R2C) {
-    const int tot = npl * H1;
+    const int tot = H1 << lg;
     for (int idx = threadIdx.x; idx < tot; idx += NT) {
-      const int p = idx / H1;
+      const int p = udiv(idx, H1, a.h1m);
       const int k = idx - p * H1;
       const int la = 2 * (p0 + p), lb = la + 1;
-      cpx<T> A = d1_load<T, D, SRC>(a, la, k);
-      cpx<T> B = lb < a.nlines ? d1_load<T, D, SRC>(a, lb, k) : cpx<T>{0, 0};
+      cpx<T> A = {(T)0, (T)0}, B = {(T)0, (T)0};
+      if (la < a.nlines) {
+        if (SRC == SRC_BUF) {
+          A = a.buf[(int64_t)la * H1 + k];
+          if (lb < a.nlines) B = a.buf[(int64_t)lb * H1 + k];
+        } else {
+          A = spec_load<D, T, SRC>(a, k, 0, 0);   // 1D: a single line
+        }
+      }
       if (k == 0 || (even && k == H1 - 1)) {  // DC / Nyquist of a real line are real
         A.y = 0;
         B.y = 0;
       }
-      cpx<T>* z = lds + p * N1;
-      z[k] = {A.x - B.y, A.y + B.x};
-      if (k > 0 && N1 - k != k) z[N1 - k] = {A.x + B.y, B.x - A.y};
+      lds[k * ps + p] = {A.x - B.y, A.y + B.x};
+      if (k > 0 && N1 - k != k) lds[(N1 - k) * ps + p] = {A.x + B.y, B.x - A.y};
     }
     __syncthreads();
-    lds_fft<T, E, false>(lds, npl, N1, a.plan, a.tw);
-    const int totr = npl * N1;
+    lds_fft<T, E, NT, false>(lds, a.plan, lg, ps, a.tw);
+    const int totr = N1 << lg;
     for (int idx = threadIdx.x; idx < totr; idx += NT) {
-      const int p = idx / N1;
-      const int n = idx - p * N1;
+      const int p = idx & (np - 1);
+      const int n = idx >> lg;
       const int la = 2 * (p0 + p), lb = la + 1;
-      cpx<T> z = lds[idx];
-      const T sa = miss_bit(a.miss, (int64_t)la * N1 + n) ? (T)0 : a.scale;
+      cpx<T> z = lds[n * ps + p];
+      const T sa = (la < a.nlines && !miss_bit(a.miss, (int64_t)la * N1 + n)) ? a.scale : (T)0;
       const T sb = (lb < a.nlines && !miss_bit(a.miss, (int64_t)lb * N1 + n)) ? a.scale : (T)0;
       z.x *= sa;
       z.y *= sb;
       if (MODE == MODE_C2R) {
-        a.real_out[(int64_t)la * N1 + n] = z.x;
+        if (la < a.nlines) a.real_out[(int64_t)la * N1 + n] = z.x;
         if (lb < a.nlines) a.real_out[(int64_t)lb * N1 + n] = z.y;
       } else {
-        lds[idx] = z;
+        lds[n * ps + p] = z;
       }
     }
     if (MODE == MODE_C2R) return;
     __syncthreads();
   } else {
-    const int totr = npl * N1;
+    const int totr = N1 << lg;
     for (int idx = threadIdx.x; idx < totr; idx += NT) {
-      const int p = idx / N1;
+      const int p = udiv(idx, N1, a.n1m);
       const int n = idx - p * N1;
       const int la = 2 * (p0 + p), lb = la + 1;
-      const T xa = a.real_in[(int64_t)la * N1 + n] * a.scale;
+      const T xa = la < a.nlines ? a.real_in[(int64_t)la * N1 + n] * a.scale : (T)0;
       const T xb = lb < a.nlines ? a.real_in[(int64_t)lb * N1 + n] * a.scale : (T)0;
-      lds[idx] = {xa, xb};
+      lds[n * ps + p] = {xa, xb};
     }
     __syncthreads();
   }
-  lds_fft<T, E, true>(lds, npl, N1, a.plan, a.tw);
+  lds_fft<T, E, NT, true>(lds, a.plan, lg, ps, a.tw);
   double acc = 0.0;
-  const int tot = npl * H1;
+  const int tot = H1 << lg;
   const T half = (T)0.5;
   for (int idx = threadIdx.x; idx < tot; idx += NT) {
-    const int p = idx / H1;
+    const int p = udiv(idx, H1, a.h1m);
     const int k = idx - p * H1;
     const int la = 2 * (p0 + p), lb = la + 1;
-    const cpx<T>* z = lds + p * N1;
-    const cpx<T> X = z[k];
-    const cpx<T> Y = z[k == 0 ? 0 : N1 - k];
+    if (la >= a.nlines) continue;
+    const cpx<T> X = lds[k * ps + p];
+    const cpx<T> Y = lds[(k == 0 ? 0 : N1 - k) * ps + p];
     const cpx<T> Va = {half * (X.x + Y.x), half * (X.y - Y.y)};
     const cpx<T> Vb = {half * (X.y + Y.y), half * (Y.x - X.x)};
     if (DST == DST_BUF) {
       a.buf[(int64_t)la * H1 + k] = Va;
       if (lb < a.nlines) a.buf[(int64_t)lb * H1 + k] = Vb;
     } else {
-      acc += scatter_elem<D, T>(a.epi, k, 0, 0, Va, a.g);   // 1D: line 0 only
+      acc += spec_store<D, T, DST>(a, k, 0, 0, Va);   // 1D: line 0 only
     }
   }
-  if (DST == DST_BETA && a.epi.dot_part) store_partial(a.epi.dot_part, acc);
+  if (DST != DST_BUF && a.epi.dot_part) store_partial(a.epi.dot_part, acc);
+}
+
+// beta <-> u (exact: a signed permutation), one thread per half-spectrum element.
+// sign = 0 moves a diagonal (L1, L2, P..) instead: the same permutation, no signs.
+template <int D, typename T>
+__global__ __launch_bounds__(256) void k_beta_to_u(const T* __restrict__ beta, cpx<T>* u,
+                                                   MapGeom g, int H1, int64_t nc, int sign) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nc) return;
+  const int k1 = (int)(e % H1);
+  const int64_t r = e / H1;
+  const int k2 = (int)(r % g.n[1]);
+  const int k3 = (int)(r / g.n[1]);
+  int64_t re, im;
+  bool vertex, neg;
+  cpx<T> w = {(T)0, (T)0};
+  if (scatter_targets<D>(k1, k2, k3, g, re, im, vertex, neg)) {
+    w.x = beta[re];
+    if (!vertex) w.y = (neg && sign) ? -beta[im] : beta[im];
+  }
+  u[e] = w;
+}
+
+template <int D, typename T>
+__global__ __launch_bounds__(256) void k_u_to_beta(const cpx<T>* __restrict__ u, T* beta,
+                                                   MapGeom g, int H1, int64_t nc) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nc) return;
+  const int k1 = (int)(e % H1);
+  const int64_t r = e / H1;
+  const int k2 = (int)(r % g.n[1]);
+  const int k3 = (int)(r / g.n[1]);
+  int64_t re, im;
+  bool vertex, neg;
+  if (scatter_targets<D>(k1, k2, k3, g, re, im, vertex, neg)) {
+    const cpx<T> w = u[e];
+    beta[re] = w.x;
+    if (!vertex) beta[im] = neg ? -w.y : w.y;
+  }
+}
+
+// bit i set when u slot i holds a beta entry (32 slots = 16 elements per thread)
+template <int D>
+__global__ __launch_bounds__(256) void k_u_valid(uint32_t* bits, MapGeom g, int H1, int64_t nc) {
+  const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (w * 16 >= nc) return;
+  uint32_t word = 0;
+  for (int j = 0; j < 16; ++j) {
+    const int64_t e = w * 16 + j;
+    if (e >= nc) break;
+    const int k1 = (int)(e % H1);
+    const int64_t r = e / H1;
+    const int k2 = (int)(r % g.n[1]);
+    const int k3 = (int)(r / g.n[1]);
+    int64_t re, im;
+    bool vertex, neg;
+    if (scatter_targets<D>(k1, k2, k3, g, re, im, vertex, neg)) {
+      word |= 1u << (2 * j);
+      if (!vertex) word |= 1u << (2 * j + 1);
+    }
+  }
+  bits[w] = word;
+}
+
+__device__ __forceinline__ bool slot_valid(const uint32_t* v, int64_t i) {
+  return (v[i >> 5] >> (i & 31)) & 1u;
 }
 
 // ===========================================================================
@@ -491,17 +614,31 @@ __device__ __forceinline__ double sum_partials(const double* part, int nparts) {
   return block_reduce(s, OpSum());
 }
 
+// The CG vectors have nb blocks of n: nb = 2 for the condensed system (the
+// block preconditioner [[P11, P12], [P12, P22]]), nb = 1 for the reduced
+// system (diagonal preconditioner P11; P12, P22 unused).
 // x = 0, r = b, z = P r, p = z; partial r'z
 template <typename T>
 __global__ __launch_bounds__(256) void k_cg_init(const T* __restrict__ b, T* x, T* r, T* z,
                                                  T* p, const T* __restrict__ P11,
                                                  const T* __restrict__ P12,
-                                                 const T* __restrict__ P22, int64_t n,
+                                                 const T* __restrict__ P22, int64_t n, int nb,
                                                  double* part) {
   double acc = 0.0;
+  const bool two = nb == 2;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
-    const T rb = b[i], rz = b[n + i];
+    const T rb = b[i];
+    if (!two) {
+      const T zb = P11[i] * rb;
+      x[i] = 0;
+      r[i] = rb;
+      z[i] = zb;
+      p[i] = zb;
+      acc += (double)rb * zb;
+      continue;
+    }
+    const T rz = b[n + i];
     const T zb = P11[i] * rb + P12[i] * rz;
     const T zz = P12[i] * rb + P22[i] * rz;
     x[i] = 0;
@@ -555,12 +692,22 @@ __global__ __launch_bounds__(256) void k_cg_update(T* x, T* r, T* z, const T* __
                                                    const T* __restrict__ P11,
                                                    const T* __restrict__ P12,
                                                    const T* __restrict__ P22, int64_t n,
-                                                   const CgState* st, double* part) {
+                                                   int nb, const CgState* st, double* part) {
   if (st->done) return;
   const T alpha = (T)st->alpha;
   double acc = 0.0;
+  const bool two = nb == 2;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
+    if (!two) {
+      x[i] += alpha * p[i];
+      const T rb = r[i] - alpha * q[i];
+      const T zb = P11[i] * rb;
+      r[i] = rb;
+      z[i] = zb;
+      acc += (double)rb * zb;
+      continue;
+    }
     x[i] += alpha * p[i];
     x[n + i] += alpha * p[n + i];
     const T rb = r[i] - alpha * q[i];
@@ -590,6 +737,54 @@ __global__ void k_cg_gamma(CgState* st, const double* part, int nparts) {
     } else if (st->iters >= st->itmax) {
       st->done = 1;
       st->status = 1;
+    }
+  }
+}
+
+// reliable update: r = b - q with q = K x just recomputed, z = P r, partial r'z
+template <typename T>
+__global__ __launch_bounds__(256) void k_cg_residual(const T* __restrict__ b,
+                                                     const T* __restrict__ q, T* r, T* z,
+                                                     const T* __restrict__ P11,
+                                                     const T* __restrict__ P12,
+                                                     const T* __restrict__ P22, int64_t n,
+                                                     int nb, const CgState* st, double* part) {
+  if (st->done) return;
+  double acc = 0.0;
+  const bool two = nb == 2;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    if (!two) {
+      const T rb = b[i] - q[i];
+      const T zb = P11[i] * rb;
+      r[i] = rb;
+      z[i] = zb;
+      acc += (double)rb * zb;
+      continue;
+    }
+    const T rb = b[i] - q[i], rz = b[n + i] - q[n + i];
+    const T zb = P11[i] * rb + P12[i] * rz;
+    const T zz = P12[i] * rb + P22[i] * rz;
+    r[i] = rb;
+    r[n + i] = rz;
+    z[i] = zb;
+    z[n + i] = zz;
+    acc += (double)rb * zb + (double)rz * zz;
+  }
+  store_partial(part, acc);
+}
+
+// after a reliable update: gamma from the true residual, restart p = z
+__global__ void k_cg_regamma(CgState* st, const double* part, int nparts) {
+  if (st->done) return;
+  const double g = sum_partials(part, nparts);
+  if (threadIdx.x == 0) {
+    st->rnorm = sqrt(g > 0 ? g : 0.0);
+    st->gamma = g;
+    st->beta = 0.0;
+    if (st->rnorm <= st->eps) {
+      st->done = 1;
+      st->status = 0;
     }
   }
 }
@@ -671,25 +866,31 @@ __global__ void k_ipm_finalize(const double* part, int nparts, RedSpec spec, dou
   if (mode == 0 && threadIdx.x == 0 && first == S_RB) scal[S_MU] = scal[S_TZ] * inv2n;
 }
 
+// slots with valid == null or bit set hold a beta entry; the others (padding
+// of the spectrum-ordered layout) stay inert: beta = 0, c = 1, z = 0
+#define FOR_SLOTS(i, n)                                                          \
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (n);      \
+       i += (int64_t)gridDim.x * blockDim.x)
+#define VALID(v, i) (!(v) || slot_valid((v), (i)))
+
 __global__ __launch_bounds__(256) void k_ipm_init(double* beta, double* c, double* z1, double* z2,
-                                                  int64_t n, const double* scal, double lam) {
+                                                  int64_t n, const double* scal, double lam,
+                                                  const uint32_t* valid) {
   const double cval = 1.0 + scal[S_BNORM];
   const double zval = lam > 0 ? 0.5 * lam : 1.0;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
+  FOR_SLOTS(i, n) {
+    const bool ok = VALID(valid, i);
     beta[i] = 0.0;
-    c[i] = cval;
-    z1[i] = zval;
-    z2[i] = zval;
+    c[i] = ok ? cval : 1.0;
+    z1[i] = ok ? zval : 0.0;
+    z2[i] = ok ? zval : 0.0;
   }
 }
 
 __global__ __launch_bounds__(256) void k_absmax(const double* __restrict__ x, int64_t n,
                                                 double* part) {
   double a[1] = {0.0};
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x)
-    a[0] = fmax(a[0], fabs(x[i]));
+  FOR_SLOTS(i, n) a[0] = fmax(a[0], fabs(x[i]));
   const int ops[1] = {1};
   store_partials<1>(part, a, ops);
 }
@@ -701,10 +902,11 @@ __global__ __launch_bounds__(256) void k_ipm_resid(const double* __restrict__ be
                                                    const double* __restrict__ z2,
                                                    const double* __restrict__ Qb,
                                                    const double* __restrict__ b, double lam,
-                                                   int64_t n, double* part) {
+                                                   int64_t n, double* part,
+                                                   const uint32_t* valid) {
   double a[6] = {0, 0, 0, 0, 0, 0};
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
+  FOR_SLOTS(i, n) {
+    if (!VALID(valid, i)) continue;
     const double be = beta[i], ce = c[i], y1 = z1[i], y2 = z2[i], q = Qb[i], bb = b[i];
     const double t1 = ce + be, t2 = ce - be;
     const double rb = q - bb - y1 + y2;
@@ -720,46 +922,103 @@ __global__ __launch_bounds__(256) void k_ipm_resid(const double* __restrict__ be
   store_partials<6>(part, a, ops);
 }
 
-// slack diagonals, preconditioner, and the predictor right-hand side
-// [-r_b + g1 - g2; -r_c + g1 + g2] with g_i = -z_i
+// slack diagonals, preconditioner (in the CG precision TC) and the predictor
+// right-hand side [-r_b + g1 - g2; -r_c + g1 + g2] with g_i = -z_i
+template <typename TC>
 __global__ __launch_bounds__(256) void k_ipm_prep(
     const double* __restrict__ beta, const double* __restrict__ c, const double* __restrict__ z1,
     const double* __restrict__ z2, const double* __restrict__ Qb, const double* __restrict__ b,
-    double lam, int64_t n, double* s1o, double* s2o, double* L1, double* L2, double* P11,
-    double* P12, double* P22, double* rhs) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const double be = beta[i], ce = c[i], y1 = z1[i], y2 = z2[i];
-    const double t1 = ce + be, t2 = ce - be;
-    const double s1 = y1 / t1, s2 = y2 / t2;
-    double l1, l2, p11, p12, p22;
-    precond_entries<double>(s1, s2, l1, l2, p11, p12, p22);
+    double lam, int64_t n, double* s1o, double* s2o, TC* L1, TC* L2, TC* P11, TC* P12, TC* P22,
+    TC* rhs, const uint32_t* valid) {
+  FOR_SLOTS(i, n) {
+    double s1 = 0, s2 = 0, rb = 0, rc = 0, y1 = 0, y2 = 0;
+    double l1 = 0, l2 = 0, p11 = 1, p12 = 0, p22 = 1;
+    const bool ok = VALID(valid, i);
+    if (ok) {
+      const double be = beta[i], ce = c[i];
+      y1 = z1[i];
+      y2 = z2[i];
+      const double t1 = ce + be, t2 = ce - be;
+      s1 = y1 / t1;
+      s2 = y2 / t2;
+      precond_entries<double>(s1, s2, l1, l2, p11, p12, p22);
+      rb = Qb[i] - b[i] - y1 + y2;
+      rc = lam - y1 - y2;
+    }
     s1o[i] = s1;
     s2o[i] = s2;
-    L1[i] = l1;
-    L2[i] = l2;
-    P11[i] = p11;
-    P12[i] = p12;
-    P22[i] = p22;
-    const double rb = Qb[i] - b[i] - y1 + y2;
-    const double rc = lam - y1 - y2;
-    rhs[i] = -rb - y1 + y2;
-    rhs[n + i] = -rc - y1 - y2;
+    L1[i] = (TC)l1;
+    L2[i] = (TC)l2;
+    P11[i] = (TC)p11;
+    P12[i] = (TC)p12;
+    P22[i] = (TC)p22;
+    rhs[i] = (TC)(ok ? -rb - y1 + y2 : 0.0);
+    rhs[n + i] = (TC)(ok ? -rc - y1 - y2 : 0.0);
   }
 }
 
+// The same for the reduced system: L1 = S1 + S2, L2 = S1 - S2 (FP64),
+// D = 4 S1 S2 / (S1 + S2) (FP64, and in TC with the preconditioner 1/(1 + D)),
+// rhsd = [h_b - (L2/L1) h_c; h_c] (FP64) and rhs = h_b - (L2/L1) h_c in TC.
+template <typename TC>
+__global__ __launch_bounds__(256) void k_ipm_prep_red(
+    const double* __restrict__ beta, const double* __restrict__ c, const double* __restrict__ z1,
+    const double* __restrict__ z2, const double* __restrict__ Qb, const double* __restrict__ b,
+    double lam, int64_t n, double* s1o, double* s2o, TC* D, TC* P, TC* rhs, double* L1d,
+    double* L2d, double* Dd, double* rhsd, const uint32_t* valid) {
+  FOR_SLOTS(i, n) {
+    double s1 = 0, s2 = 0, l1 = 0, l2 = 0, dg = 0, hb = 0, hc = 0;
+    if (VALID(valid, i)) {
+      const double be = beta[i], ce = c[i], y1 = z1[i], y2 = z2[i];
+      const double t1 = ce + be, t2 = ce - be;
+      s1 = y1 / t1;
+      s2 = y2 / t2;
+      l1 = s1 + s2;
+      l2 = s1 - s2;
+      dg = 4.0 * s1 * s2 / l1;
+      const double rb = Qb[i] - b[i] - y1 + y2;
+      const double rc = lam - y1 - y2;
+      hb = -rb - y1 + y2;
+      hc = -rc - y1 - y2;
+      hb -= (l2 / l1) * hc;
+    }
+    s1o[i] = s1;
+    s2o[i] = s2;
+    L1d[i] = l1;
+    L2d[i] = l2;
+    Dd[i] = dg;
+    D[i] = (TC)dg;
+    P[i] = (TC)(1.0 / (1.0 + dg));
+    rhsd[i] = hb;
+    rhsd[n + i] = hc;
+    rhs[i] = (TC)hb;
+  }
+}
+
+// dc = (h_c - L2 db) / L1 into the second block of d (FP64)
+__global__ __launch_bounds__(256) void k_red_back(double* d, const double* __restrict__ rhsd,
+                                                  const double* __restrict__ L1d,
+                                                  const double* __restrict__ L2d, int64_t n,
+                                                  const uint32_t* valid) {
+  FOR_SLOTS(i, n) d[n + i] = VALID(valid, i) ? (rhsd[n + i] - L2d[i] * d[i]) / L1d[i] : 0.0;
+}
+
 // affine direction: dt, dz and the largest steps to the boundary
+template <typename TC>
 __global__ __launch_bounds__(256) void k_ipm_dir_aff(
     const double* __restrict__ beta, const double* __restrict__ c, const double* __restrict__ z1,
     const double* __restrict__ z2, const double* __restrict__ s1, const double* __restrict__ s2,
-    const double* __restrict__ d, int64_t n, double* dt1o, double* dt2o, double* dz1o,
-    double* dz2o, double* part) {
+    const TC* __restrict__ d, int64_t n, double* dt1o, double* dt2o, double* dz1o, double* dz2o,
+    double* part, const uint32_t* valid) {
   double a[2] = {1e300, 1e300};
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
+  FOR_SLOTS(i, n) {
+    if (!VALID(valid, i)) {
+      dt1o[i] = dt2o[i] = dz1o[i] = dz2o[i] = 0.0;
+      continue;
+    }
     const double be = beta[i], ce = c[i], y1 = z1[i], y2 = z2[i];
     const double t1 = ce + be, t2 = ce - be;
-    const double db = d[i], dc = d[n + i];
+    const double db = (double)d[i], dc = (double)d[n + i];
     const double dt1 = dc + db, dt2 = dc - db;
     const double dz1 = -y1 - s1[i] * dt1;
     const double dz2 = -y2 - s2[i] * dt2;
@@ -781,11 +1040,11 @@ __global__ __launch_bounds__(256) void k_ipm_muaff(
     const double* __restrict__ beta, const double* __restrict__ c, const double* __restrict__ z1,
     const double* __restrict__ z2, const double* __restrict__ dt1, const double* __restrict__ dt2,
     const double* __restrict__ dz1, const double* __restrict__ dz2, const double* scal, int64_t n,
-    double* part) {
+    double* part, const uint32_t* valid) {
   const double ap = scal[S_AP_AFF], ad = scal[S_AD_AFF];
   double a[1] = {0.0};
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
+  FOR_SLOTS(i, n) {
+    if (!VALID(valid, i)) continue;
     const double be = beta[i], ce = c[i];
     const double t1 = ce + be, t2 = ce - be;
     a[0] += (t1 + ap * dt1[i]) * (z1[i] + ad * dz1[i]) + (t2 + ap * dt2[i]) * (z2[i] + ad * dz2[i]);
@@ -795,41 +1054,58 @@ __global__ __launch_bounds__(256) void k_ipm_muaff(
 }
 
 // corrector right-hand side: g_i = (sigma mu - t_i z_i - dt_i^aff dz_i^aff) / t_i
+template <typename TC>
 __global__ __launch_bounds__(256) void k_ipm_rhs_corr(
     const double* __restrict__ beta, const double* __restrict__ c, const double* __restrict__ z1,
     const double* __restrict__ z2, const double* __restrict__ Qb, const double* __restrict__ b,
     const double* __restrict__ dt1, const double* __restrict__ dt2,
     const double* __restrict__ dz1, const double* __restrict__ dz2, const double* scal,
-    double lam, int64_t n, double* rhs) {
+    double lam, int64_t n, TC* rhs, double* rhsd, const double* __restrict__ L1d,
+    const double* __restrict__ L2d, const uint32_t* valid) {
   const double smu = scal[S_SIGMA] * scal[S_MU];
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const double be = beta[i], ce = c[i], y1 = z1[i], y2 = z2[i];
-    const double t1 = ce + be, t2 = ce - be;
-    const double g1 = (smu - t1 * y1 - dt1[i] * dz1[i]) / t1;
-    const double g2 = (smu - t2 * y2 - dt2[i] * dz2[i]) / t2;
-    const double rb = Qb[i] - b[i] - y1 + y2;
-    const double rc = lam - y1 - y2;
-    rhs[i] = -rb + g1 - g2;
-    rhs[n + i] = -rc + g1 + g2;
+  FOR_SLOTS(i, n) {
+    double hb = 0, hc = 0;
+    if (VALID(valid, i)) {
+      const double be = beta[i], ce = c[i], y1 = z1[i], y2 = z2[i];
+      const double t1 = ce + be, t2 = ce - be;
+      const double g1 = (smu - t1 * y1 - dt1[i] * dz1[i]) / t1;
+      const double g2 = (smu - t2 * y2 - dt2[i] * dz2[i]) / t2;
+      const double rb = Qb[i] - b[i] - y1 + y2;
+      const double rc = lam - y1 - y2;
+      hb = -rb + g1 - g2;
+      hc = -rc + g1 + g2;
+    }
+    if (rhsd) {   // reduced system (see k_ipm_prep_red)
+      if (VALID(valid, i)) hb -= (L2d[i] / L1d[i]) * hc;
+      rhsd[i] = hb;
+      rhsd[n + i] = hc;
+      rhs[i] = (TC)hb;
+    } else {
+      rhs[i] = (TC)hb;
+      rhs[n + i] = (TC)hc;
+    }
   }
 }
 
 // corrector direction (overwrites the affine dt, dz) and boundary steps
+template <typename TC>
 __global__ __launch_bounds__(256) void k_ipm_dir(
     const double* __restrict__ beta, const double* __restrict__ c, const double* __restrict__ z1,
     const double* __restrict__ z2, const double* __restrict__ s1, const double* __restrict__ s2,
-    const double* __restrict__ d, const double* scal, int64_t n, double* dt1o, double* dt2o,
-    double* dz1o, double* dz2o, double* part) {
+    const TC* __restrict__ d, const double* scal, int64_t n, double* dt1o, double* dt2o,
+    double* dz1o, double* dz2o, double* part, const uint32_t* valid) {
   const double smu = scal[S_SIGMA] * scal[S_MU];
   double a[2] = {1e300, 1e300};
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
+  FOR_SLOTS(i, n) {
+    if (!VALID(valid, i)) {
+      dt1o[i] = dt2o[i] = dz1o[i] = dz2o[i] = 0.0;
+      continue;
+    }
     const double be = beta[i], ce = c[i], y1 = z1[i], y2 = z2[i];
     const double t1 = ce + be, t2 = ce - be;
     const double g1 = (smu - t1 * y1 - dt1o[i] * dz1o[i]) / t1;
     const double g2 = (smu - t2 * y2 - dt2o[i] * dz2o[i]) / t2;
-    const double db = d[i], dc = d[n + i];
+    const double db = (double)d[i], dc = (double)d[n + i];
     const double dt1 = dc + db, dt2 = dc - db;
     const double dz1 = g1 - s1[i] * dt1;
     const double dz2 = g2 - s2[i] * dt2;
@@ -846,21 +1122,36 @@ __global__ __launch_bounds__(256) void k_ipm_dir(
   store_partials<2>(part, a, ops);
 }
 
+template <typename TC>
 __global__ __launch_bounds__(256) void k_ipm_update(double* beta, double* c, double* z1,
-                                                    double* z2, const double* __restrict__ d,
+                                                    double* z2, const TC* __restrict__ d,
                                                     const double* __restrict__ dz1,
                                                     const double* __restrict__ dz2,
-                                                    const double* scal, int64_t n) {
+                                                    const double* scal, int64_t n,
+                                                    const uint32_t* valid) {
   const double ap = scal[S_AP], ad = scal[S_AD];
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    beta[i] += ap * d[i];
-    c[i] += ap * d[n + i];
+  FOR_SLOTS(i, n) {
+    if (!VALID(valid, i)) continue;
+    beta[i] += ap * (double)d[i];
+    c[i] += ap * (double)d[n + i];
     z1[i] += ad * dz1[i];
     z2[i] += ad * dz2[i];
   }
 }
 
+// mixed precision helpers: x_lo = (TC) x_hi;  x_hi += (double) x_lo
+template <typename TC>
+__global__ __launch_bounds__(256) void k_narrow(const double* __restrict__ x, TC* y, int64_t n) {
+  FOR_SLOTS(i, n) y[i] = (TC)x[i];
+}
+template <typename TC>
+__global__ __launch_bounds__(256) void k_widen_add(double* y, const TC* __restrict__ x, int64_t n) {
+  FOR_SLOTS(i, n) y[i] += (double)x[i];
+}
+template <typename TC>
+__global__ __launch_bounds__(256) void k_widen(double* y, const TC* __restrict__ x, int64_t n) {
+  FOR_SLOTS(i, n) y[i] = (double)x[i];
+}
 
 // ===========================================================================
 // FFTNLPModel and the full KKT operators (elementwise, op precision T)

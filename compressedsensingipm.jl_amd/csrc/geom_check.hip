@@ -3,6 +3,8 @@
 // CPU tests can compare the device index arithmetic with the oracle and check
 // every index stays inside its buffer.  No kernels; never used on the
 // product path.
+#include <vector>
+
 #include "csipm_kernels.hpp"
 
 using namespace csipm;
@@ -79,5 +81,62 @@ void geomchk_scatter(int dim, const int64_t* sizes, int64_t* re, int64_t* im, in
   if (dim == 1) sc<1>(g, re, im, flags, H1);
   else if (dim == 2) sc<2>(g, re, im, flags, H1);
   else sc<3>(g, re, im, flags, H1);
+}
+}
+
+namespace {
+template <int D>
+void via_u(const MapGeom& g, const double* beta, double* v, double* beta_back, const double* vin) {
+  const int H1 = g.n[0] / 2 + 1;
+  const int64_t nc = (int64_t)H1 * g.n[1] * g.n[2];
+  std::vector<cpx<double>> u(nc), u2(nc);
+  // beta -> u (the k_beta_to_u rule)
+  for (int64_t e = 0; e < nc; ++e) {
+    const int k1 = (int)(e % H1);
+    const int64_t r = e / H1;
+    const int k2 = (int)(r % g.n[1]), k3 = (int)(r / g.n[1]);
+    int64_t re, im;
+    bool vertex, neg;
+    cpx<double> w = {0, 0};
+    if (scatter_targets<D>(k1, k2, k3, g, re, im, vertex, neg)) {
+      w.x = beta[re];
+      if (!vertex) w.y = neg ? -beta[im] : beta[im];
+    }
+    u[e] = w;
+  }
+  // u -> v (pass A's load) and v -> u -> beta (pass E's store + k_u_to_beta)
+  for (int64_t e = 0; e < nc; ++e) {
+    const int k1 = (int)(e % H1);
+    const int64_t r = e / H1;
+    const int k2 = (int)(r % g.n[1]), k3 = (int)(r / g.n[1]);
+    const cpx<double> vv = u_to_v<D, double>(u.data(), k1, k2, k3, g, H1);
+    v[2 * e] = vv.x;
+    v[2 * e + 1] = vv.y;
+    const cpx<double> in = {vin[2 * e], vin[2 * e + 1]};
+    u2[e] = v_to_u<D, double>(in, k1, k2, k3, g);
+  }
+  for (int64_t e = 0; e < nc; ++e) {
+    const int k1 = (int)(e % H1);
+    const int64_t r = e / H1;
+    const int k2 = (int)(r % g.n[1]), k3 = (int)(r / g.n[1]);
+    int64_t re, im;
+    bool vertex, neg;
+    if (scatter_targets<D>(k1, k2, k3, g, re, im, vertex, neg)) {
+      beta_back[re] = u2[e].x;
+      if (!vertex) beta_back[im] = neg ? -u2[e].y : u2[e].y;
+    }
+  }
+}
+}  // namespace
+
+extern "C" {
+// v = u_to_v(beta_to_u(beta)) (must equal beta_to_DFT(beta));
+// beta_back = u_to_beta(v_to_u(vin)) (must equal DFT_to_beta(vin))
+void geomchk_via_u(int dim, const int64_t* sizes, const double* beta, double* v,
+                   double* beta_back, const double* vin) {
+  const MapGeom g = geom(dim, sizes);
+  if (dim == 1) via_u<1>(g, beta, v, beta_back, vin);
+  else if (dim == 2) via_u<2>(g, beta, v, beta_back, vin);
+  else via_u<3>(g, beta, v, beta_back, vin);
 }
 }

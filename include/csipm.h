@@ -37,7 +37,11 @@ enum {
   CSIPM_ERR_NOT_CONVERGED = 6 /* iterative method stopped without success */
 };
 
-enum { CSIPM_F64 = 0, CSIPM_F32 = 1 };
+/* Precision of a handle.  CSIPM_MIXED: operator and CG in FP32 (complex64)
+ * inside an FP64 interior-point driver (FP64 residuals, complementarity,
+ * step lengths, and one FP64 refinement step of every Newton direction);
+ * its operator entry points run in FP32. */
+enum { CSIPM_F64 = 0, CSIPM_F32 = 1, CSIPM_MIXED = 2 };
 
 typedef struct csipm_op* csipm_op_t;
 
@@ -52,7 +56,7 @@ const char* csipm_last_error(void);
  *   dim        1, 2 or 3 (FFTParameters.DFTdim, src/fft_utils.jl:1-14)
  *   sizes      N1, N2, N3 (unused trailing entries ignored)
  *   rdft       must be 1 (the real-to-complex path); 0 is reserved
- *   precision  CSIPM_F64 or CSIPM_F32: arithmetic of the operator apply
+ *   precision  CSIPM_F64, CSIPM_F32 or CSIPM_MIXED (see above)
  *   missing    HOST array of n_missing 0-based linear indices (index_missing
  *              of FFTParameters, converted from Julia's 1-based Int), may be
  *              NULL when n_missing == 0
@@ -134,21 +138,26 @@ int csipm_condense_rhs(csipm_op_t h, const void* w, const void* sigma1, const vo
 int csipm_unpack_solution(csipm_op_t h, void* w, const void* x, const void* sigma1,
                           const void* sigma2);
 
-/* Per-pass timing of the fused K apply (csipm_kkt_apply with alpha=1,
- * beta=0), the counterpart of FFTOperator's fft_timer / mapping_timer
- * (src/fft_utils.jl:24-25): HIP events around every pass, averaged over nrep
- * applies.  pass_ms / pass_bytes: arrays of >= 8; pass_bytes receives the
- * algorithmic HBM bytes of each pass (may be NULL).  Synchronous.         */
+/* Per-pass timing of the fused K apply (alpha=1, beta=0), the counterpart of
+ * FFTOperator's fft_timer / mapping_timer (src/fft_utils.jl:24-25): HIP
+ * events around every pass, averaged over nrep applies.
+ *   layout 0: csipm_kkt_apply on reference-ordered vectors (2N, N);
+ *          1: the apply CG runs internally, on spectrum-ordered vectors
+ *             (2*U, U with U = 2*Nc from csipm_op_info), with its fused p'q;
+ *          2: the reduced apply y = (M^T M + D) x the mixed-precision CG
+ *             runs (x, y, D = L1 of U entries; L2 unused), with its p'q.
+ *   pass_ms / pass_bytes: arrays of >= 8; pass_bytes receives the algorithmic
+ *   HBM bytes of each pass (may be NULL).  Synchronous.                   */
 int csipm_kkt_apply_timed(csipm_op_t h, const void* x, void* y, const void* L1,
-                          const void* L2, int64_t nrep, double* pass_ms,
+                          const void* L2, int64_t nrep, int layout, double* pass_ms,
                           double* pass_bytes, int* npass);
-/* Name of pass `pass` of the K apply ("" if out of range). */
-const char* csipm_kkt_pass_name(csipm_op_t h, int pass);
+/* Name of pass `pass` of the apply of that layout ("" if out of range). */
+const char* csipm_kkt_pass_name(csipm_op_t h, int layout, int pass);
 
 /* Full solve of FFTNLPModel (src/fft_model.jl:66-101) by the library's
- * primal-dual interior-point driver on the condensed FFT-KKT system.  The
- * reference delegates this loop to MadNLP (ipm_solve!,
- * src/CompressedSensingIPM.jl:31-36).
+ * primal-dual interior-point driver on the condensed FFT-KKT system (handle
+ * precision CSIPM_F64 or CSIPM_MIXED).  The reference delegates this loop to
+ * MadNLP (ipm_solve!, src/CompressedSensingIPM.jl:31-36).
  *   z0        device, N doubles (observed samples, zero at missing)
  *   lambda    l1 weight;  tol  IPM tolerance;  max_iter  IPM iteration cap
  *   cg_rtol   relative PCG tolerance per Newton system; cg_itmax cap

@@ -34,6 +34,7 @@ def glib():
     L.geomchk_beta_to_dft.argtypes = [ctypes.c_int, p, p, p, ctypes.c_int]
     L.geomchk_dft_to_beta.argtypes = [ctypes.c_int, p, p, p, ctypes.c_int]
     L.geomchk_scatter.argtypes = [ctypes.c_int, p, p, p, p]
+    L.geomchk_via_u.argtypes = [ctypes.c_int, p, p, p, p, p]
     return L
 
 
@@ -91,3 +92,27 @@ def test_scatter_is_inverse_of_gather(glib, dims):
     assert np.array_equal(np.where(hit, im, -1), oim)
     neg = (fl & 4) == 4
     assert np.array_equal(neg & (oim >= 0), (osign < 0) & (oim >= 0))
+
+
+@pytest.mark.parametrize("dims", SHAPES)
+def test_spectrum_order_is_exact(glib, dims):
+    """The spectrum-ordered copy u of beta used inside CG and the IPM: the FFT
+    passes' load (u -> v) reproduces beta_to_DFT bit-exactly, and their store
+    followed by the u -> beta conversion reproduces DFT_to_beta bit-exactly."""
+    n = int(np.prod(dims))
+    sz = _sizes(dims)
+    rng = np.random.default_rng(n + 1)
+    beta = rng.standard_normal(n)
+    h1 = dims[0] // 2 + 1
+    m = h1 * (n // dims[0])
+    x = rng.standard_normal(n)
+    spec = (np.fft.fftn(x.reshape(dims, order="F")) / np.sqrt(n))[:h1].reshape(-1, order="F")
+    vin = np.empty(2 * m)
+    vin[0::2] = spec.real
+    vin[1::2] = spec.imag
+    v = np.full(2 * m, np.nan)
+    back = np.full(n, np.nan)
+    glib.geomchk_via_u(len(dims), _p(sz), _p(beta), _p(v), _p(back), _p(vin))
+    want = maps.beta_to_dft(beta, dims, rdft=True)
+    assert np.array_equal(v[0::2], want.real) and np.array_equal(v[1::2], want.imag)
+    assert np.array_equal(back, maps.dft_to_beta(spec, dims))

@@ -24,11 +24,11 @@ def _rel(a, b):
     return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
 
 
-def _model(p, dtype=torch.float64):
+def _model(p, dtype=torch.float64, mixed=False):
     from csipm.fft_operator import FFTParameters
     from csipm.model import FFTNLPModel
     params = FFTParameters(len(p.dims), p.dims, p.z0, p.lam, p.missing)
-    return FFTNLPModel(params, rdft=True, dtype=dtype)
+    return FFTNLPModel(params, rdft=True, dtype=dtype, mixed=mixed)
 
 
 @pytest.mark.parametrize("dims", [(100,), (255, 255), (16, 12, 10)])
@@ -99,17 +99,19 @@ def _support(beta):
     return np.abs(beta) > 1e-6 * np.abs(beta).max()
 
 
-@pytest.mark.parametrize("case", ["ex1d", "ex2d", "ex3d", "cfg0", "b1d", "b2d"])
-def test_solve_vs_oracle(cuda, case):
+@pytest.mark.parametrize("mixed", [False, True])
+@pytest.mark.parametrize("case", ["ex1d", "ex2d", "ex3d", "cfg0", "b1d", "b2d", "b3d"])
+def test_solve_vs_oracle(cuda, case, mixed):
     import csipm.workloads as wl
     from csipm.model import ipm_solve
-    print("solve", case, flush=True)
+    print("solve", case, "mixed" if mixed else "f64", flush=True)
     p = {"ex1d": lambda: wl.reference_example((128,), False),
          "ex2d": lambda: wl.reference_example((24, 20), False),
          "ex3d": lambda: wl.reference_example((10, 8, 12), False),
          "cfg0": lambda: wl.config0(),
          "b1d": lambda: wl.batch_instance((1500,), seed=11),
-         "b2d": lambda: wl.batch_instance((63, 63), seed=12)}[case]()
+         "b2d": lambda: wl.batch_instance((63, 63), seed=12),
+         "b3d": lambda: wl.batch_instance((26, 21, 15), seed=13)}[case]()
     b = op.m_perpt_z(p.z0, p.dims)
     mt = lambda v: op.mtm(v, p.dims, p.missing)    # noqa: E731
     ref = ipm.solve(b, p.lam, mt, ipm.IPMOptions(tol=1e-10, cg_rtol=1e-13))
@@ -117,7 +119,7 @@ def test_solve_vs_oracle(cuda, case):
     fp, gap, primal = ipm.lasso_certificate(ref.beta, b, p.lam, mt,
                                             lambda v: op.m_perp_beta(v, p.dims, p.missing), p.z0)
     assert gap <= 1e-8 * abs(primal)
-    st = ipm_solve(_model(p), tol=1e-8)
+    st = ipm_solve(_model(p, mixed=mixed), tol=1e-8 if not mixed else 1e-7)
     assert st.status == "solved"
     beta = st.solution.cpu().numpy()
     f_gpu = ipm.objective(beta, np.abs(beta), mt(beta), b, p.lam)
