@@ -1,19 +1,29 @@
-"""Benchmark: full compressed-sensing IPM solves on one MI355X.
+"""Benchmark: compressed-sensing IPM solves on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload config1]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload config1] [--batch B]
 
-A step is one complete solve (IPM with two PCG solves per iteration, every
-operator apply through the fused HIP kernels) of one synthetic instance that
-is already resident in HBM.  `value` is solves per second over the K timed
-steps.  Also reported:
+Workload config1 (default): B instances of the 2D 255x255 generator (40
+modes, r=3 discs on a 16-lattice, lambda 0.05, tol 1e-8, instance seeds
+1..B), already resident in HBM.  A step solves all B:
 
-* roofline: the dominant pass of the fused condensed-KKT apply, timed live
-  with HIP events (csipm_kkt_apply_timed) on the stream the library launches
-  on; algorithmic bytes per launch / average duration, against 8 TB/s.
+* batched mixed (the measured form, `value`): one batch engine, all B
+  admitted at once, every kernel launch covering all of them; FP32 operator
+  and CG on the reduced Newton system, FP64 control and refinement;
+* plain FP64 (the baseline form, reported beside): the same B instances
+  solved one at a time by csipm_solve on FP64 handles (the reference's
+  condensed-K CG in FP64).
+
+`value` is solves/s of the batched mixed form over the K timed steps; the
+plain FP64 rate, the ratio, per-solve p50/p99 latencies of both forms (a
+batched solve completes with its batch) and the worst relative objective gap
+of the mixed solutions against the FP64 ones are reported beside it.  Also:
+
+* roofline: the dominant launch of the batch's CG operator apply (the
+  reduced-system M^T M + D apply over all B instances), timed live with HIP
+  events (csipm_batch_apply_timed) on the stream the engine launches on;
+  algorithmic bytes of the launch / its average duration, against 8 TB/s.
 * cpu_baseline: the FP64 CPU restatement of the reference path (oracle/,
-  scipy pocketfft) solving the same instance, run on this host's cores for a
-  bounded sample of at most ~20 s; its objective is the FP64 reference for the
-  reported objective gap.
+  scipy pocketfft) solving instance 1, a bounded sample of at most ~20 s.
 """
 import argparse
 import json
@@ -31,8 +41,9 @@ for _p in (ROOT, PKG):
 HBM_PEAK_GBS = 8000.0
 
 WORKLOADS = {
-    "config0": ("config0", "1D N=1024, 12 modes, gaps+5% dropout, lambda 0.1, tol 1e-8"),
-    "config1": ("config1", "2D 255x255, 40 modes, r=3 discs on a 16-lattice, lambda 0.05, tol 1e-8"),
+    "config0": ("config0", 1e-8, "1D N=1024, 12 modes, gaps+5% dropout, lambda 0.1, tol 1e-8"),
+    "config1": ("config1", 1e-8,
+                "2D 255x255, 40 modes, r=3 discs on a 16-lattice, lambda 0.05, tol 1e-8"),
 }
 
 
@@ -48,7 +59,7 @@ def load_traffic(workload, kernel):
         return None
 
 
-def cpu_baseline(problem, budget_s=20.0, max_runs=5):
+def cpu_baseline(problem, tol, budget_s=20.0, max_runs=5):
     import numpy as np
     from oracle import ipm
     from oracle import operator as op
@@ -60,12 +71,12 @@ def cpu_baseline(problem, budget_s=20.0, max_runs=5):
     t_start = time.perf_counter()
     while len(times) < max_runs and (time.perf_counter() - t_start) < budget_s:
         t0 = time.perf_counter()
-        res = ipm.solve(b, problem.lam, mt, ipm.IPMOptions(tol=1e-8))
+        res = ipm.solve(b, problem.lam, mt, ipm.IPMOptions(tol=tol))
         times.append(time.perf_counter() - t0)
     f_ref = ipm.objective(res.beta, np.abs(res.beta), mt(res.beta), b, problem.lam)
     med = statistics.median(times)
     return {"value": 1.0 / med, "unit": "solves/s", "cores": workers, "kind": "port",
-            "sample": f"{len(times)} FP64 oracle solve(s) of the same instance "
+            "sample": f"{len(times)} FP64 oracle solve(s) of instance 1 "
                       f"(median {med:.3f} s, scipy pocketfft workers={workers}, "
                       f"{res.iterations} IPM iterations)"}, f_ref
 
@@ -73,9 +84,10 @@ def cpu_baseline(problem, budget_s=20.0, max_runs=5):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="config1", choices=sorted(WORKLOADS))
+    ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -83,94 +95,135 @@ def main():
               "instance fits one MI355X; sharding is described in DESIGN.md)", file=sys.stderr)
         return 2
 
-    import ctypes
-
     import numpy as np
     import torch
 
     import csipm.workloads as wl
-    from csipm import _lib
-    from csipm.fft_operator import FFTParameters, _ptr
+    from csipm.batch import BatchSolver, slots_of
+    from csipm.fft_operator import FFTOperator, FFTParameters
     from csipm.model import FFTNLPModel, ipm_solve
 
     dev = torch.device("cuda:0")
-    gen, desc = WORKLOADS[args.workload]
-    problem = getattr(wl, gen)()
-    params = FFTParameters(len(problem.dims), problem.dims, problem.z0, problem.lam,
-                           problem.missing)
-    nlp = FFTNLPModel(params, rdft=True, dtype=torch.float64)
+    gen, tol, desc = WORKLOADS[args.workload]
+    B = args.batch
+    problems = [getattr(wl, gen)(instance=i + 1) for i in range(B)]
+    z0s = [torch.from_numpy(p.z0).to(dev) for p in problems]
+    print(f"# {args.workload}: {B} instances, dims {problems[0].dims}", file=sys.stderr,
+          flush=True)
+
+    models = [FFTNLPModel(FFTParameters(len(p.dims), p.dims, z, p.lam, p.missing),
+                          dtype=torch.float64) for p, z in zip(problems, z0s)]
+    mops = [FFTOperator(p.N, len(p.dims), p.dims, p.missing, mixed=True) for p in problems]
+    bs = BatchSolver(sum(slots_of(o) for o in mops), max_instances=B, mixed=True)
+
+    def run_plain():
+        out, lat = [], []
+        for m in models:
+            t0 = time.perf_counter()
+            out.append(ipm_solve(m, tol=tol))
+            torch.cuda.synchronize()
+            lat.append(time.perf_counter() - t0)
+        return out, lat
+
+    def run_batched():
+        t0 = time.perf_counter()
+        for k, (o, p, z) in enumerate(zip(mops, problems, z0s)):
+            if bs.admit(o, z, p.lam, tag=k) is None:
+                raise RuntimeError("pool too small")
+        while bs.step(tol) > 0:
+            pass
+        res = [None] * B
+        for i in bs.finished():
+            r = bs.harvest(i)
+            res[r.tag] = r
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        return res, [t1 - t0] * B
 
     for _ in range(args.warmup):
-        ipm_solve(nlp, tol=1e-8)
+        run_batched()
+        run_plain()
     torch.cuda.synchronize()
-    lat = []
-    stats = None
+    print("# warm", file=sys.stderr, flush=True)
     t_all = time.perf_counter()
+    lat_b, res_b = [], None
     for _ in range(args.steps):
-        t0 = time.perf_counter()
-        stats = ipm_solve(nlp, tol=1e-8)
-        torch.cuda.synchronize()
-        lat.append(time.perf_counter() - t0)
+        res_b, lat = run_batched()
+        lat_b += lat
     torch.cuda.synchronize()
-    total = time.perf_counter() - t_all
+    total_b = time.perf_counter() - t_all
+    print(f"# batched mixed: {total_b:.3f} s", file=sys.stderr, flush=True)
+    t_all = time.perf_counter()
+    lat_p, res_p = [], None
+    for _ in range(args.steps):
+        res_p, lat = run_plain()
+        lat_p += lat
+    torch.cuda.synchronize()
+    total_p = time.perf_counter() - t_all
+    print(f"# plain fp64: {total_p:.3f} s", file=sys.stderr, flush=True)
 
-    # roofline of the dominant pass of the fused K apply
-    # the K apply the CG runs (spectrum-ordered vectors of length U = 2*Nc)
-    n = 2 * nlp.op_fft.Nc
-    dt = nlp.op_fft.dtype
-    rng = np.random.default_rng(0)
-    x = torch.from_numpy(rng.standard_normal(2 * n)).to(dev, dt)
-    L1 = torch.from_numpy(rng.uniform(0.1, 3, n)).to(dev, dt)
-    L2 = torch.from_numpy(rng.uniform(-1, 1, n)).to(dev, dt)
-    y = torch.empty(2 * n, dtype=dt, device=dev)
-    ms = (ctypes.c_double * 8)()
-    by = (ctypes.c_double * 8)()
-    npass = ctypes.c_int()
-    h = nlp.op_fft.handle
-    _lib.call("csipm_kkt_apply_timed", h, _ptr(x), _ptr(y), _ptr(L1), _ptr(L2), 20, 1, ms, by,
-              ctypes.byref(npass))
-    _lib.call("csipm_kkt_apply_timed", h, _ptr(x), _ptr(y), _ptr(L1), _ptr(L2), 200, 1, ms, by,
-              ctypes.byref(npass))
-    passes = [(_lib.lib().csipm_kkt_pass_name(h, 1, i).decode(), ms[i], by[i])
-              for i in range(npass.value)]
-    name, pms, pbytes = max(passes, key=lambda p: p[1])
-    achieved = pbytes / (pms * 1e-3) / 1e9
-    kapply_ms = sum(p[1] for p in passes)
+    gaps = [abs(rb.objective - rp.objective) / abs(rp.objective) for rb, rp in zip(res_b, res_p)]
+    beta_err = [float(np.linalg.norm(rb.beta.cpu().numpy() - rp.solution.cpu().numpy()) /
+                      np.linalg.norm(rp.solution.cpu().numpy()))
+                for rb, rp in zip(res_b[:8], res_p[:8])]
+    statuses = [r.status for r in res_b] + [r.status for r in res_p]
+
+    # roofline: the dominant launch of the batch's CG operator apply
+    for k, (o, p, z) in enumerate(zip(mops, problems, z0s)):
+        bs.admit(o, z, p.lam, tag=k)
+    bs.apply_timed(5)
+    launches = bs.apply_timed(50)
+    name, lms, lbytes = max(launches, key=lambda t: t[1])
+    achieved = lbytes / (lms * 1e-3) / 1e9
+    apply_ms = sum(t[1] for t in launches)
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                "traffic": load_traffic(args.workload, name),
-                "kernel": name, "kernel_ms": round(pms, 5), "kernel_bytes": int(pbytes),
-                "k_apply_ms": round(kapply_ms, 5),
-                "passes": {p[0]: {"ms": round(p[1], 5), "bytes": int(p[2])} for p in passes}}
+                "traffic": load_traffic(f"{args.workload}-batch{B}-mixed", name),
+                "kernel": name, "kernel_ms": round(lms, 5), "kernel_bytes": int(lbytes),
+                "apply_ms": round(apply_ms, 5),
+                "apply_GBps": round(sum(t[2] for t in launches) / (apply_ms * 1e-3) / 1e9, 1),
+                "launches": {t[0]: {"ms": round(t[1], 5), "bytes": int(t[2])} for t in launches}}
 
+    n_solves = B * args.steps
+    value = n_solves / total_b
+    plain = n_solves / total_p
+    p0 = problems[0]
     out = {
         "metric": "solves/s at <=1e-6 obj gap, batched FP32-op vs plain FP64; p50/p99 latency",
-        "value": round(args.steps / total, 4),
+        "value": round(value, 3),
         "unit": "solves/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(1e3 * total / args.steps, 3),
+        "ms_per_step": round(1e3 * total_b / args.steps, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32-op/f64-control",
         "data": "synthetic",
-        "config": {"workload": f"{problem.name}: {desc}; plain FP64 one solve at a time",
-                   "dims": list(problem.dims), "N": problem.N, "missing": int(len(problem.missing)),
-                   "lambda": problem.lam},
-        "p50_ms": round(1e3 * statistics.median(lat), 3),
-        "p99_ms": round(1e3 * float(np.percentile(lat, 99)), 3),
-        "ipm_iterations": stats.iterations,
-        "cg_iterations": stats.cg_iterations,
-        "objective": stats.objective,
+        "config": {"workload": f"{args.workload}: {desc}; a step = {B} instances solved "
+                               "batched (FP32 operator/CG, FP64 control) in one engine",
+                   "dims": list(p0.dims), "N": p0.N, "batch": B, "lambda": p0.lam, "tol": tol},
+        "plain_fp64": {"value": round(plain, 3), "unit": "solves/s",
+                       "p50_ms": round(1e3 * statistics.median(lat_p), 3),
+                       "p99_ms": round(1e3 * float(np.percentile(lat_p, 99)), 3),
+                       "ipm_iterations": res_p[0].iterations,
+                       "cg_iterations": res_p[0].cg_iterations},
+        "batched_over_plain": round(value / plain, 3),
+        "p50_ms": round(1e3 * statistics.median(lat_b), 3),
+        "p99_ms": round(1e3 * float(np.percentile(lat_b, 99)), 3),
+        "ipm_iterations": res_b[0].iterations,
+        "cg_iterations": res_b[0].cg_iterations,
+        "max_obj_rel_gap_vs_fp64": max(gaps),
+        "beta_rel_l2_vs_fp64": max(beta_err),
+        "all_solved": all(s == "solved" for s in statuses),
         "roofline": roofline,
     }
+    bs.close()
     if not args.no_cpu_baseline:
-        cb, f_ref = cpu_baseline(problem)
+        cb, f_ref = cpu_baseline(problems[0], tol)
         out["cpu_baseline"] = cb
-        out["objective_ref"] = f_ref
-        out["obj_rel_gap"] = abs(stats.objective - f_ref) / abs(f_ref)
+        out["obj_rel_gap_fp64_vs_cpu"] = abs(res_p[0].objective - f_ref) / abs(f_ref)
     print(json.dumps(out), flush=True)
     return 0
 

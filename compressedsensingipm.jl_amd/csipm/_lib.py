@@ -40,6 +40,20 @@ SIGNATURES = {
     "csipm_cg_solve": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _d, _i64,
                             ctypes.POINTER(_i64), ctypes.POINTER(_d), ctypes.POINTER(_i)]),
     "csipm_solve": (_i, [_vp, _vp, _d, _d, _i64, _d, _i64, _vp, ctypes.POINTER(_d)]),
+    "csipm_batch_create": (_i, [ctypes.POINTER(_vp), _i, _i64, _i, _vp]),
+    "csipm_batch_destroy": (_i, [_vp]),
+    "csipm_batch_slots": (_i, [_vp, ctypes.POINTER(_i64)]),
+    "csipm_batch_bytes": (_i, [_i, _i64, ctypes.POINTER(_i64)]),
+    "csipm_batch_admit": (_i, [_vp, _vp, _vp, _d, _vp, _i64, ctypes.POINTER(_i)]),
+    "csipm_batch_step": (_i, [_vp, _d, _i64, _d, ctypes.POINTER(_i)]),
+    "csipm_batch_poll": (_i, [_vp, ctypes.POINTER(_i), _i, ctypes.POINTER(_i)]),
+    "csipm_batch_harvest": (_i, [_vp, _i, ctypes.POINTER(_d)]),
+    "csipm_batch_apply_timed": (_i, [_vp, _i64, ctypes.POINTER(_d), ctypes.POINTER(_d), _i,
+                                     ctypes.POINTER(_i)]),
+    "csipm_batch_launch_name": (ctypes.c_char_p, [_vp, _i]),
+    "csipm_batch_solve": (_i, [_vp, _i, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                               ctypes.POINTER(_d), _d, _i64, _d, _i64, ctypes.POINTER(_vp),
+                               ctypes.POINTER(_d)]),
     "csipm_nlp_obj": (_i, [_vp, _vp, _vp, _d, ctypes.POINTER(_d)]),
     "csipm_nlp_grad": (_i, [_vp, _vp, _vp, _d, _vp]),
     "csipm_nlp_cons": (_i, [_vp, _vp, _vp]),

@@ -9,10 +9,11 @@
 #include <cstring>
 #include <initializer_list>
 #include <string>
+#include <complex>
 #include <vector>
 
 #include "csipm.h"
-#include "csipm_kernels.hpp"
+#include "csipm_batch.hpp"
 
 using namespace csipm;
 
@@ -67,9 +68,14 @@ struct Dim1Cfg {
   size_t lds = 0;
 };
 
+struct Engine;
+
 // per precision: twiddles and launch shapes (the plans are shared)
 struct PrecState {
   void* tw[3] = {nullptr, nullptr, nullptr};
+  void* chirp[3] = {nullptr, nullptr, nullptr};   // Bluestein tables (axes with bs)
+  void* bfw[3] = {nullptr, nullptr, nullptr};
+  void* binv[3] = {nullptr, nullptr, nullptr};
   LineCfg lines[3];
   Dim1Cfg d1;
 };
@@ -84,7 +90,8 @@ struct csipm_op {
   MapGeom g{};
   int H1 = 0;
   int64_t N = 0, Nc = 0, U = 0;  // U = 2*Nc: length of the spectrum-ordered copy of beta
-  FftPlan plan[3]{};
+  FftPlan plan[3]{};             // line FFT of each axis (length M for Bluestein axes)
+  bool bs[3] = {false, false, false};
   PrecState p64, p32;
   uint32_t* miss = nullptr;
   int64_t nmiss = 0;
@@ -94,16 +101,11 @@ struct csipm_op {
   int npart = 0;
   CgState* cg = nullptr;         // device
   CgState* cg_host = nullptr;    // pinned mirror
-  void* cgw = nullptr;           // CG vectors r, z, p, q (2*max(N, U) each, op precision)
-  void* conv = nullptr;          // reference-layout CG: u copies of b, x, L1, L2, P11, P12, P22
-  double* ipm = nullptr;         // interior-point workspace
-  void* ipm_lo = nullptr;        // its CG-precision part
-  double* scal = nullptr;
-  double* scal_host = nullptr;
+  void* scratch = nullptr;       // N-vector scratch of the NLP entry points (op precision)
+  Engine* solo = nullptr;        // batch-of-one engine of csipm_solve / csipm_cg_solve
   bool has64() const { return prec == CSIPM_F64 || prec == CSIPM_MIXED; }
   bool has32() const { return prec == CSIPM_F32 || prec == CSIPM_MIXED; }
   size_t esize() const { return prec == CSIPM_F64 ? 8 : 4; }   // op / CG precision
-  int64_t vlen() const { return std::max<int64_t>(N, U); }
   template <typename T>
   PrecState& ps();
 };
@@ -149,6 +151,76 @@ static bool factorize(int L, FftPlan& p) {
     ns *= p.radix[s];
   }
   return true;
+}
+
+// axis plan: the direct mixed-radix plan when L factors into the supported
+// radices, else Bluestein with the power-of-two convolution length M >= 2L-1
+static bool plan_axis(int L, FftPlan& p, bool& bs) {
+  bs = false;
+  if (factorize(L, p)) return true;
+  int M = 1;
+  while (M < 2 * L - 1) M <<= 1;
+  bs = true;
+  return factorize(M, p);
+}
+
+// in-place forward DFT (exp(-2 pi i jk/M)) of a power-of-two length, host FP64
+static void host_fft_pow2(std::vector<std::complex<double>>& a) {
+  const size_t M = a.size();
+  for (size_t i = 1, j = 0; i < M; ++i) {
+    size_t bit = M >> 1;
+    for (; j & bit; bit >>= 1) j ^= bit;
+    j ^= bit;
+    if (i < j) std::swap(a[i], a[j]);
+  }
+  const long double pi = acosl(-1.0L);
+  for (size_t len = 2; len <= M; len <<= 1) {
+    const size_t half = len / 2;
+    for (size_t k = 0; k < half; ++k) {
+      const long double ang = -2.0L * pi * (long double)k / (long double)len;
+      const std::complex<double> w((double)cosl(ang), (double)sinl(ang));
+      for (size_t i = 0; i < M; i += len) {
+        const std::complex<double> u = a[i + k], v = a[i + k + half] * w;
+        a[i + k] = u + v;
+        a[i + k + half] = u - v;
+      }
+    }
+  }
+}
+
+// Bluestein tables of line length L, convolution length M: the chirp
+// w_n = exp(-i pi (n^2 mod 2L) / L) and the kernel spectra FFT_M(b)/M of the
+// forward (b = conj w) and inverse (b = w) convolution kernels, wrapped so
+// that b_{M-m} = b_m
+template <typename T>
+static int upload_bluestein(int L, int M, void** chirp, void** bfw, void** binv) {
+  const long double pi = acosl(-1.0L);
+  std::vector<std::complex<double>> w(L);
+  for (int n = 0; n < L; ++n) {
+    const long long q = ((long long)n * n) % (2LL * L);
+    const long double ang = -pi * (long double)q / (long double)L;
+    w[n] = std::complex<double>((double)cosl(ang), (double)sinl(ang));
+  }
+  std::vector<cpx<T>> hc(L);
+  for (int n = 0; n < L; ++n) hc[n] = {(T)w[n].real(), (T)w[n].imag()};
+  HIPCHK(hipMalloc(chirp, sizeof(cpx<T>) * (size_t)L));
+  HIPCHK(hipMemcpy(*chirp, hc.data(), sizeof(cpx<T>) * (size_t)L, hipMemcpyHostToDevice));
+  for (int dir = 0; dir < 2; ++dir) {
+    std::vector<std::complex<double>> b((size_t)M, 0.0);
+    for (int m = 0; m < L; ++m) {
+      const std::complex<double> v = dir == 0 ? std::conj(w[m]) : w[m];
+      b[(size_t)m] = v;
+      if (m > 0) b[(size_t)(M - m)] = v;
+    }
+    host_fft_pow2(b);
+    std::vector<cpx<T>> hb((size_t)M);
+    for (int m = 0; m < M; ++m)
+      hb[(size_t)m] = {(T)(b[(size_t)m].real() / M), (T)(b[(size_t)m].imag() / M)};
+    void** out = dir == 0 ? bfw : binv;
+    HIPCHK(hipMalloc(out, sizeof(cpx<T>) * (size_t)M));
+    HIPCHK(hipMemcpy(*out, hb.data(), sizeof(cpx<T>) * (size_t)M, hipMemcpyHostToDevice));
+  }
+  return CSIPM_OK;
 }
 
 template <typename T>
@@ -228,11 +300,17 @@ PassArgs<T> base_args(csipm_op* h, int axis) {
   a.h1m = magic(h->H1);
   a.n1m = magic(h->n[0]);
   a.miss = h->miss;
+  a.len = h->n[axis];
+  a.bs = h->bs[axis] ? 1 : 0;
+  a.chirp = (const cpx<T>*)h->ps<T>().chirp[axis];
+  a.bfw = (const cpx<T>*)h->ps<T>().bfw[axis];
+  a.binv = (const cpx<T>*)h->ps<T>().binv[axis];
   return a;
 }
 
-template <typename T, int D, int SRC, int DST, bool FWD>
-int launch_lines(csipm_op* h, int axis, const void* src, const Epi<T>& epi, const int* done) {
+template <typename T>
+PassArgs<T> lines_args(csipm_op* h, int axis, const void* src, const Epi<T>& epi,
+                       const int* done) {
   const LineCfg& c = h->ps<T>().lines[axis];
   PassArgs<T> a = base_args<T>(h, axis);
   a.axis = axis;
@@ -243,16 +321,12 @@ int launch_lines(csipm_op* h, int axis, const void* src, const Epi<T>& epi, cons
   a.u_in = (const cpx<T>*)src;
   a.epi = epi;
   a.done = done;
-  auto kern = k_lines<T, kNT, kE, D, SRC, DST, FWD>;
-  CHK(allow_lds(kern, c.lds));
-  hipLaunchKernelGGL(kern, dim3(c.grid), dim3(kNT), c.lds, h->stream, a);
-  HIPCHK(hipGetLastError());
-  return CSIPM_OK;
+  return a;
 }
 
-template <typename T, int D, int SRC, int DST, int MODE>
-int launch_dim1(csipm_op* h, const void* src, const T* real_in, T* real_out, const Epi<T>& epi,
-                T scale, const int* done) {
+template <typename T>
+PassArgs<T> dim1_args(csipm_op* h, const void* src, const T* real_in, T* real_out,
+                      const Epi<T>& epi, T scale, const int* done) {
   const Dim1Cfg& c = h->ps<T>().d1;
   PassArgs<T> a = base_args<T>(h, 0);
   a.lg = c.lg;
@@ -266,6 +340,25 @@ int launch_dim1(csipm_op* h, const void* src, const T* real_in, T* real_out, con
   a.epi = epi;
   a.scale = scale;
   a.done = done;
+  return a;
+}
+
+template <typename T, int D, int SRC, int DST, bool FWD>
+int launch_lines(csipm_op* h, int axis, const void* src, const Epi<T>& epi, const int* done) {
+  const LineCfg& c = h->ps<T>().lines[axis];
+  PassArgs<T> a = lines_args<T>(h, axis, src, epi, done);
+  auto kern = k_lines<T, kNT, kE, D, SRC, DST, FWD>;
+  CHK(allow_lds(kern, c.lds));
+  hipLaunchKernelGGL(kern, dim3(c.grid), dim3(kNT), c.lds, h->stream, a);
+  HIPCHK(hipGetLastError());
+  return CSIPM_OK;
+}
+
+template <typename T, int D, int SRC, int DST, int MODE>
+int launch_dim1(csipm_op* h, const void* src, const T* real_in, T* real_out, const Epi<T>& epi,
+                T scale, const int* done) {
+  const Dim1Cfg& c = h->ps<T>().d1;
+  PassArgs<T> a = dim1_args<T>(h, src, real_in, real_out, epi, scale, done);
   if (c.nt == kNT) {
     auto kern = k_dim1<T, kNT, kE, D, SRC, DST, MODE>;
     CHK(allow_lds(kern, c.lds));
@@ -444,294 +537,73 @@ int final_grid(csipm_op* h) {
   return h->ps<T>().lines[h->dim - 1].grid;
 }
 
-// ---------------------------------------------------------------------------
-// preconditioned CG (Krylov.jl cg recurrence), device-resident scalars.
-// nb = 2: the condensed system K = [[M^T M + L1, L2], [L2, L1]] on vectors of
-// 2n entries with the block preconditioner [[P11, P12], [P12, P22]];
-// nb = 1: the reduced system M^T M + D (D passed as L1) on n entries with the
-// diagonal preconditioner P11.  reliable > 0: every `reliable` iterations the
-// residual is recomputed as b - K x and the recurrence restarts from it.
-// ---------------------------------------------------------------------------
-template <typename T, int SP>
-int cg_apply(csipm_op* h, int nb, const T* x, T* y, const T* L1, const T* L2, double* dot_part,
-             const int* done) {
-  if (nb == 2) return kkt_apply_t<T, SP>(h, x, y, L1, L2, 1.0, 0.0, dot_part, done);
-  return red_apply_t<T, SP>(h, x, y, L1, 1.0, 0.0, dot_part, done);
-}
+#include "csipm_engine.hpp"
 
-template <typename T, int SP>
-int cg_run(csipm_op* h, int nb, const T* b, T* x, const T* L1, const T* L2, const T* P11,
-           const T* P12, const T* P22, double atol, double rtol, int64_t itmax, int reliable,
-           int64_t* iters, double* resid, int* status) {
-  const int64_t n = SP == SP_U ? h->U : h->N;
-  const int64_t vl = 2 * h->vlen();
-  T* r = (T*)h->cgw;
-  T* z = r + vl;
-  T* p = z + vl;
-  T* q = p + vl;
-  CgState init{};
-  init.atol = atol;
-  init.rtol = rtol;
-  init.itmax = itmax > 0 ? itmax : 4 * h->N;
-  HIPCHK(hipMemcpyAsync(h->cg, &init, sizeof(CgState), hipMemcpyHostToDevice, h->stream));
-  const int gv = grid_for(n, 4);
-  hipLaunchKernelGGL((k_cg_init<T>), dim3(gv), dim3(kNT), 0, h->stream, b, x, r, z, p, P11, P12,
-                     P22, n, nb, h->part);
-  hipLaunchKernelGGL(k_cg_start, dim3(1), dim3(kNT), 0, h->stream, h->cg, h->part, gv);
-  HIPCHK(hipGetLastError());
-  const int* done = &h->cg->done;
-  const int gk = final_grid<T>(h);
-  const int check_every = 4;
-  for (int64_t it = 0;; ++it) {
-    if (it % check_every == 0) {
-      HIPCHK(hipMemcpyAsync(h->cg_host, h->cg, sizeof(CgState), hipMemcpyDeviceToHost, h->stream));
-      HIPCHK(hipStreamSynchronize(h->stream));
-      if (h->cg_host->done) break;
-    }
-    CHK((cg_apply<T, SP>(h, nb, p, q, L1, L2, h->part, done)));
-    hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(kNT), 0, h->stream, h->cg, h->part, gk);
-    hipLaunchKernelGGL((k_cg_update<T>), dim3(gv), dim3(kNT), 0, h->stream, x, r, z, p, q, P11,
-                       P12, P22, n, nb, h->cg, h->part);
-    hipLaunchKernelGGL(k_cg_gamma, dim3(1), dim3(kNT), 0, h->stream, h->cg, h->part, gv);
-    if (reliable > 0 && (it + 1) % reliable == 0) {
-      CHK((cg_apply<T, SP>(h, nb, x, q, L1, L2, nullptr, done)));
-      hipLaunchKernelGGL((k_cg_residual<T>), dim3(gv), dim3(kNT), 0, h->stream, b, q, r, z, P11,
-                         P12, P22, n, nb, h->cg, h->part);
-      hipLaunchKernelGGL(k_cg_regamma, dim3(1), dim3(kNT), 0, h->stream, h->cg, h->part, gv);
-    }
-    hipLaunchKernelGGL((k_cg_pupdate<T>), dim3(gv), dim3(kNT), 0, h->stream, p, z, nb * n, h->cg);
-    HIPCHK(hipGetLastError());
+// the handle's batch-of-one engine, created on first use, sized for it
+int solo_engine(csipm_op* h, Engine** out) {
+  if (!h->solo) {
+    const bool mixed = h->prec == CSIPM_MIXED;
+    CHK(engine_create(&h->solo, slots_of(h), 1, (int)h->esize(), mixed, h->stream));
   }
-  *iters = h->cg_host->iters;
-  *resid = h->cg_host->rnorm;
-  *status = h->cg_host->status;
+  h->solo->stream = h->stream;
+  *out = h->solo;
   return CSIPM_OK;
 }
 
-// beta-layout <-> u conversions of one N-vector
-// sign = 1 for a vector, 0 for a diagonal matrix (whose entries keep their sign)
+// csipm_cg_solve: the reference-layout system converted to the spectrum
+// order, solved there by the batched CG (one instance), converted back
 template <typename T>
-int to_u(csipm_op* h, const T* beta, T* u, int sign = 1) {
-  const int grid = (int)((h->Nc + kNT - 1) / kNT);
-  cpx<T>* uu = (cpx<T>*)u;
-  if (h->dim == 1)
-    hipLaunchKernelGGL((k_beta_to_u<1, T>), dim3(grid), dim3(kNT), 0, h->stream, beta, uu, h->g, h->H1, h->Nc, sign);
-  else if (h->dim == 2)
-    hipLaunchKernelGGL((k_beta_to_u<2, T>), dim3(grid), dim3(kNT), 0, h->stream, beta, uu, h->g, h->H1, h->Nc, sign);
-  else
-    hipLaunchKernelGGL((k_beta_to_u<3, T>), dim3(grid), dim3(kNT), 0, h->stream, beta, uu, h->g, h->H1, h->Nc, sign);
-  HIPCHK(hipGetLastError());
-  return CSIPM_OK;
+int cg_solve_solo(csipm_op* h, const T* b, T* x, const T* L1, const T* L2, const T* P11,
+                  const T* P12, const T* P22, double atol, double rtol, int64_t itmax,
+                  int64_t* iters, double* resid, int* status) {
+  Engine* E = nullptr;
+  CHK(solo_engine(h, &E));
+  int id = -1;
+  CHK(engine_admit(E, h, nullptr, 0.0, nullptr, itmax, &id));
+  const int64_t N = h->N, U = h->U, off = E->inst[(size_t)id].off;
+  hipStream_t st = E->stream;
+  int rc = CSIPM_OK;
+  do {
+    T* ub = E->C2<T>(C2_RHS) + 2 * off;
+    if ((rc = to_u<T>(h, b, ub, st)) || (rc = to_u<T>(h, b + N, ub + U, st))) break;
+    const T* diag[5] = {L1, L2, P11, P12, P22};
+    const int slot[5] = {C_L1, C_L2, C_P11, C_P12, C_P22};
+    for (int k = 0; k < 5 && !rc; ++k) rc = to_u<T>(h, diag[k], E->C<T>(slot[k]) + off, st, 0);
+    if (rc) break;
+    if ((rc = engine_tables<T>(E, 2, false))) break;
+    if ((rc = engine_cg<T>(E, 2, atol, rtol, 0))) break;
+    T* ux = E->C2<T>(C2_D) + 2 * off;
+    if ((rc = from_u<T>(h, ux, x, st)) || (rc = from_u<T>(h, ux + U, x + N, st))) break;
+    if (hipStreamSynchronize(st) != hipSuccess) {
+      rc = fail(CSIPM_ERR_HIP, "CG failed");
+      break;
+    }
+    const CgState& cs = E->cg_h[id];
+    *iters = cs.iters;
+    *resid = cs.rnorm;
+    *status = cs.status;
+  } while (false);
+  engine_drop(E, id);
+  return rc;
 }
 
-template <typename T>
-int from_u(csipm_op* h, const T* u, T* beta) {
-  const int grid = (int)((h->Nc + kNT - 1) / kNT);
-  const cpx<T>* uu = (const cpx<T>*)u;
-  if (h->dim == 1)
-    hipLaunchKernelGGL((k_u_to_beta<1, T>), dim3(grid), dim3(kNT), 0, h->stream, uu, beta, h->g, h->H1, h->Nc);
-  else if (h->dim == 2)
-    hipLaunchKernelGGL((k_u_to_beta<2, T>), dim3(grid), dim3(kNT), 0, h->stream, uu, beta, h->g, h->H1, h->Nc);
-  else
-    hipLaunchKernelGGL((k_u_to_beta<3, T>), dim3(grid), dim3(kNT), 0, h->stream, uu, beta, h->g, h->H1, h->Nc);
-  HIPCHK(hipGetLastError());
-  return CSIPM_OK;
-}
-
-// the reference-layout CG (csipm_cg_solve): converted to the spectrum order,
-// solved there, converted back
-template <typename T>
-int cg_solve_ref(csipm_op* h, const T* b, T* x, const T* L1, const T* L2, const T* P11,
-                 const T* P12, const T* P22, double atol, double rtol, int64_t itmax,
-                 int64_t* iters, double* resid, int* status) {
-  const int64_t N = h->N, U = h->U;
-  if (!h->conv) HIPCHK(hipMalloc(&h->conv, sizeof(T) * (size_t)U * 9));
-  T* ub = (T*)h->conv;
-  T* ux = ub + 2 * U;
-  T* uL1 = ux + 2 * U;
-  T* uL2 = uL1 + U;
-  T* uP11 = uL2 + U;
-  T* uP12 = uP11 + U;
-  T* uP22 = uP12 + U;
-  CHK(to_u<T>(h, b, ub));
-  CHK(to_u<T>(h, b + N, ub + U));
-  CHK(to_u<T>(h, L1, uL1, 0));
-  CHK(to_u<T>(h, L2, uL2, 0));
-  CHK(to_u<T>(h, P11, uP11, 0));
-  CHK(to_u<T>(h, P12, uP12, 0));
-  CHK(to_u<T>(h, P22, uP22, 0));
-  CHK((cg_run<T, SP_U>(h, 2, ub, ux, uL1, uL2, uP11, uP12, uP22, atol, rtol, itmax, 0, iters,
-                       resid, status)));
-  CHK(from_u<T>(h, ux, x));
-  CHK(from_u<T>(h, ux + U, x + N));
-  return CSIPM_OK;
-}
-
-// ---------------------------------------------------------------------------
-// interior-point driver: Mehrotra predictor-corrector on the condensed FFT-KKT
-// system, all vectors in the spectrum order (u), control in FP64.  TC is the
-// precision of the K apply inside CG (double, or float with one FP64
-// refinement step of each Newton direction).
-// ---------------------------------------------------------------------------
-enum { W_B = 0, W_BETA, W_C, W_Z1, W_Z2, W_QB, W_S1, W_S2, W_DT1, W_DT2, W_DZ1, W_DZ2,
-       W_L1D, W_L2D, W_DGD, W_NVEC, W_RHSD = W_NVEC, W_DD = W_NVEC + 2, W_TMP = W_NVEC + 4,
-       W_TOTAL = W_NVEC + 6 };
-enum { C_L1 = 0, C_L2, C_P11, C_P12, C_P22, C_NVEC, C_RHS = C_NVEC, C_D = C_NVEC + 2,
-       C_TOTAL = C_NVEC + 4 };
-
-int ipm_finalize(csipm_op* h, int nparts, std::initializer_list<int> ops, int mode, int first,
-                 double tau) {
-  RedSpec spec{};
-  spec.nq = 0;
-  for (int op : ops) spec.op[spec.nq++] = op;
-  hipLaunchKernelGGL(k_ipm_finalize, dim3(1), dim3(kNT), 0, h->stream, h->part, nparts, spec,
-                     h->scal, mode, first, 0.5 / (double)h->N, tau);
-  HIPCHK(hipGetLastError());
-  return CSIPM_OK;
-}
-
+// csipm_solve: the interior-point driver on a batch of one
 template <typename TC>
-int ipm_solve_t(csipm_op* h, const double* z0, double lam, double tol, int64_t max_iter,
-                double cg_rtol, int64_t cg_itmax, double* beta_out, double* info) {
+int solve_solo(csipm_op* h, const double* z0, double lam, double tol, int64_t max_iter,
+               double cg_rtol, int64_t cg_itmax, double* beta_out, double* info) {
   const auto t0 = std::chrono::steady_clock::now();
-  const int64_t U = h->U;
-  const bool mixed = sizeof(TC) == 4;
-  if (!h->ipm) {
-    HIPCHK(hipMalloc(&h->ipm, sizeof(double) * (size_t)U * W_TOTAL));
-    HIPCHK(hipMalloc(&h->ipm_lo, sizeof(TC) * (size_t)U * C_TOTAL));
-    HIPCHK(hipMalloc(&h->scal, sizeof(double) * S_NSLOTS));
-    HIPCHK(hipHostMalloc(&h->scal_host, sizeof(double) * S_NSLOTS, hipHostMallocDefault));
+  Engine* E = nullptr;
+  CHK(solo_engine(h, &E));
+  int id = -1;
+  CHK(engine_admit(E, h, z0, lam, beta_out, cg_itmax, &id));
+  const int mi = (int)std::min<int64_t>(max_iter, 1 << 30);
+  int rc = engine_run<TC>(E, tol, mi, cg_rtol);
+  if (rc) {
+    engine_drop(E, id);
+    return rc;
   }
-  double* W[W_TOTAL];
-  for (int i = 0; i < W_TOTAL; ++i) W[i] = h->ipm + (size_t)i * U;
-  TC* C[C_TOTAL];
-  for (int i = 0; i < C_TOTAL; ++i) C[i] = (TC*)h->ipm_lo + (size_t)i * U;
-  double *b = W[W_B], *beta = W[W_BETA], *c = W[W_C], *z1 = W[W_Z1], *z2 = W[W_Z2], *Qb = W[W_QB];
-  double *s1 = W[W_S1], *s2 = W[W_S2], *L1d = W[W_L1D], *L2d = W[W_L2D];
-  double *dt1 = W[W_DT1], *dt2 = W[W_DT2], *dz1 = W[W_DZ1], *dz2 = W[W_DZ2];
-  double *rhsd = W[W_RHSD], *dd = W[W_DD], *tmp = W[W_TMP], *Dgd = W[W_DGD];
-  TC *L1 = C[C_L1], *L2 = C[C_L2], *P11 = C[C_P11], *P12 = C[C_P12], *P22 = C[C_P22];
-  TC *rhs = C[C_RHS], *d = C[C_D];
-  const uint32_t* valid = h->valid;
-  const int gv = grid_for(U, 4);
-  const double tau = 0.995;
-  const double rtol = mixed ? std::max(cg_rtol, 1e-5) : cg_rtol;
-  const int reliable = mixed ? 25 : 0;
-  // b = M^T z0, in the spectrum order
-  CHK((dispatch_m_perpt_z<double, SP_U>(h, z0, b)));
-  hipLaunchKernelGGL(k_absmax, dim3(gv), dim3(kNT), 0, h->stream, b, U, h->part);
-  CHK(ipm_finalize(h, gv, {1}, 0, S_BNORM, tau));
-  hipLaunchKernelGGL(k_ipm_init, dim3(gv), dim3(kNT), 0, h->stream, beta, c, z1, z2, U, h->scal,
-                     lam, valid);
-  HIPCHK(hipGetLastError());
-  int64_t cg_total = 0;
-  int status = 1;
-  double f = 0, dual = 0, mu = 0;
-  int64_t it = 0;
-  // One Newton solve.  FP64: CG on the condensed system (the reference's
-  // K), direction in d.  Mixed: the c-block is eliminated in FP64
-  // (dc = (h_c - L2 db) / L1 leaves (M^T M + D) db = h_b - (L2/L1) h_c with
-  // D = L1 - L2^2/L1 = 4 S1 S2 / (S1 + S2) > 0), the reduced system is solved
-  // by FP32 CG with the diagonal preconditioner 1/(1 + D), then one FP64
-  // refinement step; direction in dd.  In FP32 the condensed K itself is
-  // unusable near the solution: L1 and |L2| agree to more than 24 bits there.
-  auto newton = [&]() -> int {
-    int64_t its;
-    double res;
-    int st;
-    if (!mixed) {
-      CHK((cg_run<TC, SP_U>(h, 2, rhs, d, L1, L2, P11, P12, P22, 0.0, rtol, cg_itmax, reliable,
-                            &its, &res, &st)));
-      cg_total += its;
-      return CSIPM_OK;
-    }
-    CHK((cg_run<TC, SP_U>(h, 1, rhs, d, L1, nullptr, P11, nullptr, nullptr, 0.0, rtol, cg_itmax,
-                          reliable, &its, &res, &st)));
-    cg_total += its;
-    hipLaunchKernelGGL((k_widen<TC>), dim3(gv), dim3(kNT), 0, h->stream, dd, d, U);
-    HIPCHK(hipMemcpyAsync(tmp, rhsd, sizeof(double) * U, hipMemcpyDeviceToDevice, h->stream));
-    CHK((red_apply_t<double, SP_U>(h, dd, tmp, Dgd, -1.0, 1.0, nullptr, nullptr)));
-    hipLaunchKernelGGL((k_narrow<TC>), dim3(gv), dim3(kNT), 0, h->stream, tmp, rhs, U);
-    CHK((cg_run<TC, SP_U>(h, 1, rhs, d, L1, nullptr, P11, nullptr, nullptr, 0.0, rtol, cg_itmax,
-                          reliable, &its, &res, &st)));
-    cg_total += its;
-    hipLaunchKernelGGL((k_widen_add<TC>), dim3(gv), dim3(kNT), 0, h->stream, dd, d, U);
-    hipLaunchKernelGGL(k_red_back, dim3(gv), dim3(kNT), 0, h->stream, dd, rhsd, L1d, L2d, U,
-                       valid);
-    HIPCHK(hipGetLastError());
-    return CSIPM_OK;
-  };
-  for (it = 0; it <= max_iter; ++it) {
-    CHK((dispatch_mtm<double, SP_U>(h, beta, epi_store<double>(Qb), nullptr)));
-    hipLaunchKernelGGL(k_ipm_resid, dim3(gv), dim3(kNT), 0, h->stream, beta, c, z1, z2, Qb, b, lam,
-                       U, h->part, valid);
-    CHK(ipm_finalize(h, gv, {1, 1, 0, 0, 0, 0}, 0, S_RB, tau));
-    HIPCHK(hipMemcpyAsync(h->scal_host, h->scal, sizeof(double) * S_NSLOTS,
-                          hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(hipStreamSynchronize(h->stream));
-    const double* sc = h->scal_host;
-    dual = std::max(sc[S_RB], sc[S_RC]);
-    mu = sc[S_MU];
-    f = 0.5 * sc[S_BQB] - sc[S_BB] + lam * sc[S_SUMC];
-    if (!std::isfinite(f) || !std::isfinite(mu) || !std::isfinite(dual)) {
-      status = 2;
-      break;
-    }
-    if (dual <= tol * (1.0 + sc[S_BNORM]) && mu <= tol * (1.0 + std::fabs(f) / (double)h->N)) {
-      status = 0;
-      break;
-    }
-    if (it == max_iter) break;
-    if (mixed)
-      hipLaunchKernelGGL((k_ipm_prep_red<TC>), dim3(gv), dim3(kNT), 0, h->stream, beta, c, z1, z2,
-                         Qb, b, lam, U, s1, s2, L1, P11, rhs, L1d, L2d, Dgd, rhsd, valid);
-    else
-      hipLaunchKernelGGL((k_ipm_prep<TC>), dim3(gv), dim3(kNT), 0, h->stream, beta, c, z1, z2, Qb,
-                         b, lam, U, s1, s2, L1, L2, P11, P12, P22, rhs, valid);
-    HIPCHK(hipGetLastError());
-    CHK(newton());
-    if (mixed)
-      hipLaunchKernelGGL((k_ipm_dir_aff<double>), dim3(gv), dim3(kNT), 0, h->stream, beta, c, z1,
-                         z2, s1, s2, dd, U, dt1, dt2, dz1, dz2, h->part, valid);
-    else
-      hipLaunchKernelGGL((k_ipm_dir_aff<TC>), dim3(gv), dim3(kNT), 0, h->stream, beta, c, z1, z2,
-                         s1, s2, d, U, dt1, dt2, dz1, dz2, h->part, valid);
-    CHK(ipm_finalize(h, gv, {2, 2}, 1, 0, tau));
-    hipLaunchKernelGGL(k_ipm_muaff, dim3(gv), dim3(kNT), 0, h->stream, beta, c, z1, z2, dt1, dt2,
-                       dz1, dz2, h->scal, U, h->part, valid);
-    CHK(ipm_finalize(h, gv, {0}, 2, 0, tau));
-    hipLaunchKernelGGL((k_ipm_rhs_corr<TC>), dim3(gv), dim3(kNT), 0, h->stream, beta, c, z1, z2,
-                       Qb, b, dt1, dt2, dz1, dz2, h->scal, lam, U, rhs, mixed ? rhsd : nullptr,
-                       L1d, L2d, valid);
-    HIPCHK(hipGetLastError());
-    CHK(newton());
-    if (mixed) {
-      hipLaunchKernelGGL((k_ipm_dir<double>), dim3(gv), dim3(kNT), 0, h->stream, beta, c, z1, z2,
-                         s1, s2, dd, h->scal, U, dt1, dt2, dz1, dz2, h->part, valid);
-      CHK(ipm_finalize(h, gv, {2, 2}, 3, 0, tau));
-      hipLaunchKernelGGL((k_ipm_update<double>), dim3(gv), dim3(kNT), 0, h->stream, beta, c, z1,
-                         z2, dd, dz1, dz2, h->scal, U, valid);
-    } else {
-      hipLaunchKernelGGL((k_ipm_dir<TC>), dim3(gv), dim3(kNT), 0, h->stream, beta, c, z1, z2, s1,
-                         s2, d, h->scal, U, dt1, dt2, dz1, dz2, h->part, valid);
-      CHK(ipm_finalize(h, gv, {2, 2}, 3, 0, tau));
-      hipLaunchKernelGGL((k_ipm_update<TC>), dim3(gv), dim3(kNT), 0, h->stream, beta, c, z1, z2,
-                         d, dz1, dz2, h->scal, U, valid);
-    }
-    HIPCHK(hipGetLastError());
-  }
-  CHK(from_u<double>(h, beta, beta_out));
-  HIPCHK(hipStreamSynchronize(h->stream));
-  const double secs =
-      std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-  info[0] = f;
-  info[1] = (double)it;
-  info[2] = (double)cg_total;
-  info[3] = (double)status;
-  info[4] = dual;
-  info[5] = mu;
-  info[6] = secs;
-  info[7] = 0.0;
+  CHK(engine_harvest(E, id, info));
+  info[6] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   return CSIPM_OK;
 }
 
@@ -748,31 +620,31 @@ const char* csipm_last_error(void) { return g_err.c_str(); }
 
 int csipm_op_destroy(csipm_op_t h) {
   if (!h) return CSIPM_OK;
-  for (int i = 0; i < 3; ++i) {
-    if (h->p64.tw[i]) (void)hipFree(h->p64.tw[i]);
-    if (h->p32.tw[i]) (void)hipFree(h->p32.tw[i]);
-  }
-  void* bufs[] = {h->miss, h->valid, h->buf, h->part, h->cg, h->cgw, h->conv,
-                  h->ipm, h->ipm_lo, h->scal};
+  for (int i = 0; i < 3; ++i)
+    for (PrecState* ps : {&h->p64, &h->p32})
+      for (void* b : {ps->tw[i], ps->chirp[i], ps->bfw[i], ps->binv[i]})
+        if (b) (void)hipFree(b);
+  delete h->solo;
+  void* bufs[] = {h->miss, h->valid, h->buf, h->part, h->scratch};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
-  if (h->cg_host) (void)hipHostFree(h->cg_host);
-  if (h->scal_host) (void)hipHostFree(h->scal_host);
   delete h;
   return CSIPM_OK;
 }
 
 static int set_shapes(csipm_op* h, PrecState& ps, size_t cs) {
   const int dim = h->dim;
-  ps.d1 = dim1_cfg(h->n[0], (int)(h->N / h->n[0]), cs);
-  if (ps.d1.nt * kE < h->n[0] || ps.d1.lds > kLdsMax)
-    return fail(CSIPM_ERR_UNSUPPORTED, "dimension-1 length %d exceeds one workgroup", h->n[0]);
-  if (dim >= 2) ps.lines[1] = line_cfg(h->H1, h->n[1], h->n[2], cs);
-  if (dim == 3) ps.lines[2] = line_cfg(h->H1, h->n[2], h->n[1], cs);
+  const int F0 = h->plan[0].L;   // FFT length (line length, or Bluestein's M)
+  ps.d1 = dim1_cfg(F0, (int)(h->N / h->n[0]), cs);
+  if (ps.d1.nt * kE < F0 || ps.d1.lds > kLdsMax)
+    return fail(CSIPM_ERR_UNSUPPORTED, "dimension-1 length %d (FFT length %d) exceeds one "
+                "workgroup", h->n[0], F0);
+  if (dim >= 2) ps.lines[1] = line_cfg(h->H1, h->plan[1].L, h->n[2], cs);
+  if (dim == 3) ps.lines[2] = line_cfg(h->H1, h->plan[2].L, h->n[1], cs);
   for (int ax = 1; ax < dim; ++ax)
-    if (h->n[ax] > kNT * kE || ps.lines[ax].lds > kLdsMax)
-      return fail(CSIPM_ERR_UNSUPPORTED, "dimension-%d length %d exceeds one workgroup", ax + 1,
-                  h->n[ax]);
+    if (h->plan[ax].L > kNT * kE || ps.lines[ax].lds > kLdsMax)
+      return fail(CSIPM_ERR_UNSUPPORTED, "dimension-%d length %d (FFT length %d) exceeds one "
+                  "workgroup", ax + 1, h->n[ax], h->plan[ax].L);
   return CSIPM_OK;
 }
 
@@ -799,11 +671,19 @@ static int create_impl(csipm_op* h, int dim, const int64_t sizes[3], int rdft, i
   h->Nc = (int64_t)h->H1 * h->n[1] * h->n[2];
   h->U = 2 * h->Nc;
   for (int i = 0; i < dim; ++i) {
-    if (!factorize(h->n[i], h->plan[i]))
-      return fail(CSIPM_ERR_UNSUPPORTED,
-                  "size %d has a prime factor above 17 (Bluestein not yet implemented)", h->n[i]);
-    if (h->has64()) CHK(upload_twiddles<double>(h->n[i], &h->p64.tw[i]));
-    if (h->has32()) CHK(upload_twiddles<float>(h->n[i], &h->p32.tw[i]));
+    if (!plan_axis(h->n[i], h->plan[i], h->bs[i]))
+      return fail(CSIPM_ERR_UNSUPPORTED, "no FFT plan for size %d", h->n[i]);
+    const int F = h->plan[i].L;
+    if (h->has64()) CHK(upload_twiddles<double>(F, &h->p64.tw[i]));
+    if (h->has32()) CHK(upload_twiddles<float>(F, &h->p32.tw[i]));
+    if (h->bs[i]) {
+      if (h->has64())
+        CHK(upload_bluestein<double>(h->n[i], F, &h->p64.chirp[i], &h->p64.bfw[i],
+                                     &h->p64.binv[i]));
+      if (h->has32())
+        CHK(upload_bluestein<float>(h->n[i], F, &h->p32.chirp[i], &h->p32.bfw[i],
+                                    &h->p32.binv[i]));
+    }
   }
   if (h->has64()) CHK(set_shapes(h, h->p64, sizeof(cpx<double>)));
   if (h->has32()) CHK(set_shapes(h, h->p32, sizeof(cpx<float>)));
@@ -846,9 +726,6 @@ static int create_impl(csipm_op* h, int dim, const int64_t sizes[3], int rdft, i
   }
   h->npart = 8 * np;   // up to 8 reduced quantities per pass
   HIPCHK(hipMalloc(&h->part, sizeof(double) * (size_t)h->npart));
-  HIPCHK(hipMalloc(&h->cg, sizeof(CgState)));
-  HIPCHK(hipHostMalloc(&h->cg_host, sizeof(CgState), hipHostMallocDefault));
-  HIPCHK(hipMalloc(&h->cgw, h->esize() * (size_t)(8 * h->vlen())));
   HIPCHK(hipStreamSynchronize(h->stream));
   return CSIPM_OK;
 }
@@ -993,12 +870,12 @@ int csipm_cg_solve(csipm_op_t h, const void* b, void* x, const void* L1, const v
   if (!h || !b || !x || !L1 || !L2 || !P11 || !P12 || !P22 || !iters || !resid || !status)
     return fail(CSIPM_ERR_ARG, "null argument");
   if (OP_DOUBLE(h))
-    return cg_solve_ref<double>(h, (const double*)b, (double*)x, (const double*)L1,
-                                (const double*)L2, (const double*)P11, (const double*)P12,
-                                (const double*)P22, atol, rtol, itmax, iters, resid, status);
-  return cg_solve_ref<float>(h, (const float*)b, (float*)x, (const float*)L1, (const float*)L2,
-                             (const float*)P11, (const float*)P12, (const float*)P22, atol, rtol,
-                             itmax, iters, resid, status);
+    return cg_solve_solo<double>(h, (const double*)b, (double*)x, (const double*)L1,
+                                 (const double*)L2, (const double*)P11, (const double*)P12,
+                                 (const double*)P22, atol, rtol, itmax, iters, resid, status);
+  return cg_solve_solo<float>(h, (const float*)b, (float*)x, (const float*)L1, (const float*)L2,
+                              (const float*)P11, (const float*)P12, (const float*)P22, atol,
+                              rtol, itmax, iters, resid, status);
 }
 
 int csipm_solve(csipm_op_t h, const double* z0, double lambda, double tol, int64_t max_iter,
@@ -1007,9 +884,9 @@ int csipm_solve(csipm_op_t h, const double* z0, double lambda, double tol, int64
   if (!(lambda >= 0) || !(tol > 0) || max_iter < 0)
     return fail(CSIPM_ERR_ARG, "bad lambda / tol / max_iter");
   if (h->prec == CSIPM_F64)
-    return ipm_solve_t<double>(h, z0, lambda, tol, max_iter, cg_rtol, cg_itmax, beta_out, info);
+    return solve_solo<double>(h, z0, lambda, tol, max_iter, cg_rtol, cg_itmax, beta_out, info);
   if (h->prec == CSIPM_MIXED)
-    return ipm_solve_t<float>(h, z0, lambda, tol, max_iter, cg_rtol, cg_itmax, beta_out, info);
+    return solve_solo<float>(h, z0, lambda, tol, max_iter, cg_rtol, cg_itmax, beta_out, info);
   return fail(CSIPM_ERR_UNSUPPORTED, "csipm_solve needs a CSIPM_F64 or CSIPM_MIXED handle");
 }
 
@@ -1018,7 +895,8 @@ int csipm_solve(csipm_op_t h, const double* z0, double lambda, double tol, int64
 extern "C++" {
 template <typename T>
 static int nlp_obj_t(csipm_op* h, const T* x, const T* mtz0, double lam, double* f) {
-  T* y = (T*)h->cgw;   // N-vector scratch
+  if (!h->scratch) HIPCHK(hipMalloc(&h->scratch, h->esize() * (size_t)h->N));
+  T* y = (T*)h->scratch;
   int rc;
   if (h->dim == 1) rc = apply_m_perp_beta<T, 1>(h, x, y);
   else if (h->dim == 2) rc = apply_m_perp_beta<T, 2>(h, x, y);
@@ -1052,7 +930,8 @@ static int nlp_grad_t(csipm_op* h, const T* x, const T* mtz0, double lam, T* g) 
 
 template <typename T>
 static int kkt_full_t(csipm_op* h, const T* x, T* y, double alpha, double beta) {
-  T* Mb = (T*)h->cgw;
+  if (!h->scratch) HIPCHK(hipMalloc(&h->scratch, h->esize() * (size_t)h->N));
+  T* Mb = (T*)h->scratch;
   CHK((dispatch_mtm<T, SP_BETA>(h, x, epi_store<T>(Mb), nullptr)));
   hipLaunchKernelGGL((k_kkt_full<T>), dim3(grid_for(h->N, 4)), dim3(kNT), 0, h->stream, x, y, Mb,
                      (T)alpha, (T)beta, h->N);
@@ -1183,6 +1062,140 @@ const char* csipm_kkt_pass_name(csipm_op_t h, int layout, int pass) {
   if (!h || layout < 0 || layout > 2) return "";
   const int np = kkt_pass_info(h, layout, names, bytes);
   return (pass >= 0 && pass < np) ? names[pass] : "";
+}
+
+// ---------------------------------------------------------------------------
+// batch engine
+// ---------------------------------------------------------------------------
+struct csipm_batch {
+  Engine* E = nullptr;
+  int prec = CSIPM_F64;
+  std::vector<std::string> names;   // of the last csipm_batch_apply_timed
+};
+
+#define BATCH_TC(b, EXPR_D, EXPR_F) ((b)->prec == CSIPM_F64 ? (EXPR_D) : (EXPR_F))
+
+int csipm_batch_create(csipm_batch_t* out, int precision, int64_t slots, int max_instances,
+                       void* stream) {
+  if (!out) return fail(CSIPM_ERR_ARG, "null argument");
+  *out = nullptr;
+  if (precision != CSIPM_F64 && precision != CSIPM_MIXED)
+    return fail(CSIPM_ERR_ARG, "batch precision must be CSIPM_F64 or CSIPM_MIXED");
+  if (slots < 1 || max_instances < 1) return fail(CSIPM_ERR_ARG, "empty batch");
+  csipm_batch* b = new (std::nothrow) csipm_batch();
+  if (!b) return fail(CSIPM_ERR_ALLOC, "out of host memory");
+  b->prec = precision;
+  const int rc = engine_create(&b->E, slots, max_instances, precision == CSIPM_F64 ? 8 : 4,
+                               precision == CSIPM_MIXED, (hipStream_t)stream);
+  if (rc) {
+    delete b;
+    return rc;
+  }
+  *out = b;
+  return CSIPM_OK;
+}
+
+int csipm_batch_destroy(csipm_batch_t b) {
+  if (!b) return CSIPM_OK;
+  delete b->E;
+  delete b;
+  return CSIPM_OK;
+}
+
+int csipm_batch_slots(csipm_op_t h, int64_t* slots) {
+  if (!h || !slots) return fail(CSIPM_ERR_ARG, "null argument");
+  *slots = slots_of(h);
+  return CSIPM_OK;
+}
+
+int csipm_batch_bytes(int precision, int64_t slots, int64_t* bytes) {
+  if (!bytes) return fail(CSIPM_ERR_ARG, "null argument");
+  *bytes = (int64_t)engine_bytes_per_slot(precision == CSIPM_F64 ? 8 : 4) * slots;
+  return CSIPM_OK;
+}
+
+int csipm_batch_admit(csipm_batch_t b, csipm_op_t h, const double* z0, double lambda,
+                      double* beta_out, int64_t cg_itmax, int* id) {
+  if (!b || !h || !z0 || !beta_out || !id) return fail(CSIPM_ERR_ARG, "null argument");
+  if (h->prec != b->prec) return fail(CSIPM_ERR_ARG, "handle precision differs from the batch");
+  if (!(lambda >= 0)) return fail(CSIPM_ERR_ARG, "bad lambda");
+  return engine_admit(b->E, h, z0, lambda, beta_out, cg_itmax, id);
+}
+
+int csipm_batch_step(csipm_batch_t b, double tol, int64_t max_iter, double cg_rtol,
+                     int* running) {
+  if (!b || !running) return fail(CSIPM_ERR_ARG, "null argument");
+  if (!(tol > 0) || max_iter < 0) return fail(CSIPM_ERR_ARG, "bad tol / max_iter");
+  const int mi = (int)std::min<int64_t>(max_iter, 1 << 30);
+  return BATCH_TC(b, engine_step<double>(b->E, tol, mi, cg_rtol, running),
+                  engine_step<float>(b->E, tol, mi, cg_rtol, running));
+}
+
+int csipm_batch_poll(csipm_batch_t b, int* ids, int cap, int* n) {
+  if (!b || !n || (cap > 0 && !ids)) return fail(CSIPM_ERR_ARG, "null argument");
+  Engine* E = b->E;
+  HIPCHK(hipMemcpyAsync(E->flags_h, E->flags, sizeof(BFlags) * (size_t)E->maxinst,
+                        hipMemcpyDeviceToHost, E->stream));
+  HIPCHK(hipStreamSynchronize(E->stream));
+  int k = 0;
+  for (int id : E->active)
+    if (!E->inst[(size_t)id].fresh && E->flags_h[id].done) {
+      if (k < cap) ids[k] = id;
+      ++k;
+    }
+  *n = k;
+  return CSIPM_OK;
+}
+
+int csipm_batch_harvest(csipm_batch_t b, int id, double* info) {
+  if (!b) return fail(CSIPM_ERR_ARG, "null argument");
+  Engine* E = b->E;
+  if (id < 0 || id >= E->maxinst || !E->inst[(size_t)id].h || E->inst[(size_t)id].fresh)
+    return fail(CSIPM_ERR_ARG, "no admitted instance %d", id);
+  return engine_harvest(E, id, info);
+}
+
+int csipm_batch_solve(csipm_batch_t b, int n, const csipm_op_t* ops, const double* const* z0,
+                      const double* lambda, double tol, int64_t max_iter, double cg_rtol,
+                      int64_t cg_itmax, double* const* beta_out, double* info) {
+  if (!b || n < 0 || (n > 0 && (!ops || !z0 || !lambda || !beta_out || !info)))
+    return fail(CSIPM_ERR_ARG, "null argument");
+  std::vector<int> ids((size_t)n, -1);
+  for (int i = 0; i < n; ++i) {
+    const int rc = csipm_batch_admit(b, ops[i], z0[i], lambda[i], beta_out[i], cg_itmax, &ids[i]);
+    if (rc) {
+      for (int j = 0; j < i; ++j) engine_drop(b->E, ids[(size_t)j]);
+      return rc;
+    }
+  }
+  for (;;) {
+    int running = 0;
+    const int rc = csipm_batch_step(b, tol, max_iter, cg_rtol, &running);
+    if (rc) {
+      for (int i = 0; i < n; ++i) engine_drop(b->E, ids[(size_t)i]);
+      return rc;
+    }
+    if (running == 0) break;
+  }
+  for (int i = 0; i < n; ++i) CHK(engine_harvest(b->E, ids[(size_t)i], info + 8 * (size_t)i));
+  return CSIPM_OK;
+}
+
+int csipm_batch_apply_timed(csipm_batch_t b, int64_t nrep, double* pass_ms,
+                            double* pass_bytes, int cap, int* nlaunch) {
+  if (!b || !pass_ms || !pass_bytes || !nlaunch || nrep < 1)
+    return fail(CSIPM_ERR_ARG, "bad argument");
+  b->names.assign((size_t)std::max(cap, 0), std::string());
+  return BATCH_TC(b,
+                  engine_apply_timed<double>(b->E, nrep, pass_ms, pass_bytes, b->names.data(),
+                                             cap, nlaunch),
+                  engine_apply_timed<float>(b->E, nrep, pass_ms, pass_bytes, b->names.data(),
+                                            cap, nlaunch));
+}
+
+const char* csipm_batch_launch_name(csipm_batch_t b, int k) {
+  if (!b || k < 0 || k >= (int)b->names.size()) return "";
+  return b->names[(size_t)k].c_str();
 }
 
 }  // extern "C"

@@ -1,0 +1,864 @@
+// Batch engine of libcsipm: the interior-point driver and its CG over any set
+// of operator handles at once.  Host code, part of csipm.hip's translation
+// unit (included inside its anonymous namespace, after the pass launchers).
+//
+// Instances live in slot ranges of one pool of spectrum-ordered vectors (see
+// csipm_batch.hpp); every kernel launch covers all instances it is given, so
+// a batch of uneven 1D/2D/3D problems costs the launches of one.  The
+// operator apply of the batch is a chain of passes per dimensionality: each
+// pass is one launch over the tiles of all instances of that dimensionality
+// (split only where instances need different workgroup sizes), reading its
+// per-instance arguments from a device table.  A single solve is a batch of
+// one.  Instances are admitted and retired between interior-point iterations
+// (continuous batching): their slots, partial sums and state are reused.
+
+// ---------------------------------------------------------------------------
+// beta-layout <-> u conversions of one N-vector of instance h
+// sign = 1 for a vector, 0 for a diagonal matrix (whose entries keep their sign)
+// ---------------------------------------------------------------------------
+template <typename T>
+int to_u(csipm_op* h, const T* beta, T* u, hipStream_t st, int sign = 1) {
+  const int grid = (int)((h->Nc + kNT - 1) / kNT);
+  cpx<T>* uu = (cpx<T>*)u;
+  if (h->dim == 1)
+    hipLaunchKernelGGL((k_beta_to_u<1, T>), dim3(grid), dim3(kNT), 0, st, beta, uu, h->g, h->H1, h->Nc, sign);
+  else if (h->dim == 2)
+    hipLaunchKernelGGL((k_beta_to_u<2, T>), dim3(grid), dim3(kNT), 0, st, beta, uu, h->g, h->H1, h->Nc, sign);
+  else
+    hipLaunchKernelGGL((k_beta_to_u<3, T>), dim3(grid), dim3(kNT), 0, st, beta, uu, h->g, h->H1, h->Nc, sign);
+  HIPCHK(hipGetLastError());
+  return CSIPM_OK;
+}
+
+template <typename T>
+int from_u(csipm_op* h, const T* u, T* beta, hipStream_t st) {
+  const int grid = (int)((h->Nc + kNT - 1) / kNT);
+  const cpx<T>* uu = (const cpx<T>*)u;
+  if (h->dim == 1)
+    hipLaunchKernelGGL((k_u_to_beta<1, T>), dim3(grid), dim3(kNT), 0, st, uu, beta, h->g, h->H1, h->Nc);
+  else if (h->dim == 2)
+    hipLaunchKernelGGL((k_u_to_beta<2, T>), dim3(grid), dim3(kNT), 0, st, uu, beta, h->g, h->H1, h->Nc);
+  else
+    hipLaunchKernelGGL((k_u_to_beta<3, T>), dim3(grid), dim3(kNT), 0, st, uu, beta, h->g, h->H1, h->Nc);
+  HIPCHK(hipGetLastError());
+  return CSIPM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// operator chains
+// ---------------------------------------------------------------------------
+enum { PK_LINES = 0, PK_DIM1 = 1 };
+struct PassSpec {
+  int kind, D, src, dst, fm, axis;  // fm: FWD (lines) or MODE (dim1)
+};
+
+// MTM: the fused M^T M pass sequence on u (with the epilogue in the last
+// pass); MTZ: M^T z from real samples into u
+enum { CH_MTM = 0, CH_MTZ = 1 };
+int chain_specs(int dim, int which, PassSpec* s) {
+  if (which == CH_MTM) {
+    if (dim == 1) { s[0] = {PK_DIM1, 1, SRC_U, DST_U, MODE_MTM, 0}; return 1; }
+    if (dim == 2) {
+      s[0] = {PK_LINES, 2, SRC_U, DST_BUF, 0, 1};
+      s[1] = {PK_DIM1, 2, SRC_BUF, DST_BUF, MODE_MTM, 0};
+      s[2] = {PK_LINES, 2, SRC_BUF, DST_U, 1, 1};
+      return 3;
+    }
+    s[0] = {PK_LINES, 3, SRC_U, DST_BUF, 0, 2};
+    s[1] = {PK_LINES, 3, SRC_BUF, DST_BUF, 0, 1};
+    s[2] = {PK_DIM1, 3, SRC_BUF, DST_BUF, MODE_MTM, 0};
+    s[3] = {PK_LINES, 3, SRC_BUF, DST_BUF, 1, 1};
+    s[4] = {PK_LINES, 3, SRC_BUF, DST_U, 1, 2};
+    return 5;
+  }
+  if (dim == 1) { s[0] = {PK_DIM1, 1, SRC_REAL, DST_U, MODE_R2C, 0}; return 1; }
+  if (dim == 2) {
+    s[0] = {PK_DIM1, 2, SRC_REAL, DST_BUF, MODE_R2C, 0};
+    s[1] = {PK_LINES, 2, SRC_BUF, DST_U, 1, 1};
+    return 2;
+  }
+  s[0] = {PK_DIM1, 3, SRC_REAL, DST_BUF, MODE_R2C, 0};
+  s[1] = {PK_LINES, 3, SRC_BUF, DST_BUF, 1, 1};
+  s[2] = {PK_LINES, 3, SRC_BUF, DST_U, 1, 2};
+  return 3;
+}
+
+// the batched kernel of one pass spec (the combinations the chains use)
+template <typename T, int NT>
+int launch_spec(const PassSpec& s, int grid, size_t lds, hipStream_t st, const void* tabv,
+                BSeg seg) {
+  const PassArgs<T>* tab = (const PassArgs<T>*)tabv;
+#define CSIPM_LAUNCH(kern)                                                      \
+  do {                                                                          \
+    CHK(allow_lds(kern, lds));                                                  \
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(NT), lds, st, tab, seg);          \
+    HIPCHK(hipGetLastError());                                                  \
+    return CSIPM_OK;                                                            \
+  } while (0)
+  if (s.kind == PK_LINES) {
+    if (NT != kNT) return fail(CSIPM_ERR_UNSUPPORTED, "line pass with %d threads", NT);
+    if (s.D == 2 && s.src == SRC_U && s.dst == DST_BUF && !s.fm)
+      CSIPM_LAUNCH((kb_lines<T, kNT, kE, 2, SRC_U, DST_BUF, false>));
+    if (s.D == 2 && s.src == SRC_BUF && s.dst == DST_U && s.fm)
+      CSIPM_LAUNCH((kb_lines<T, kNT, kE, 2, SRC_BUF, DST_U, true>));
+    if (s.D == 3 && s.src == SRC_U && s.dst == DST_BUF && !s.fm)
+      CSIPM_LAUNCH((kb_lines<T, kNT, kE, 3, SRC_U, DST_BUF, false>));
+    if (s.D == 3 && s.src == SRC_BUF && s.dst == DST_BUF && !s.fm)
+      CSIPM_LAUNCH((kb_lines<T, kNT, kE, 3, SRC_BUF, DST_BUF, false>));
+    if (s.D == 3 && s.src == SRC_BUF && s.dst == DST_BUF && s.fm)
+      CSIPM_LAUNCH((kb_lines<T, kNT, kE, 3, SRC_BUF, DST_BUF, true>));
+    if (s.D == 3 && s.src == SRC_BUF && s.dst == DST_U && s.fm)
+      CSIPM_LAUNCH((kb_lines<T, kNT, kE, 3, SRC_BUF, DST_U, true>));
+  } else {
+    if (s.D == 1 && s.src == SRC_U && s.fm == MODE_MTM)
+      CSIPM_LAUNCH((kb_dim1<T, NT, kE, 1, SRC_U, DST_U, MODE_MTM>));
+    if (s.D == 1 && s.src == SRC_REAL && s.fm == MODE_R2C)
+      CSIPM_LAUNCH((kb_dim1<T, NT, kE, 1, SRC_REAL, DST_U, MODE_R2C>));
+    if (s.D == 2 && s.src == SRC_BUF && s.fm == MODE_MTM)
+      CSIPM_LAUNCH((kb_dim1<T, NT, kE, 2, SRC_BUF, DST_BUF, MODE_MTM>));
+    if (s.D == 2 && s.src == SRC_REAL && s.fm == MODE_R2C)
+      CSIPM_LAUNCH((kb_dim1<T, NT, kE, 2, SRC_REAL, DST_BUF, MODE_R2C>));
+    if (s.D == 3 && s.src == SRC_BUF && s.fm == MODE_MTM)
+      CSIPM_LAUNCH((kb_dim1<T, NT, kE, 3, SRC_BUF, DST_BUF, MODE_MTM>));
+    if (s.D == 3 && s.src == SRC_REAL && s.fm == MODE_R2C)
+      CSIPM_LAUNCH((kb_dim1<T, NT, kE, 3, SRC_REAL, DST_BUF, MODE_R2C>));
+  }
+#undef CSIPM_LAUNCH
+  return fail(CSIPM_ERR_UNSUPPORTED, "no batched kernel for pass (%d,%d,%d,%d,%d)", s.kind, s.D,
+              s.src, s.dst, s.fm);
+}
+
+// one launch of a chain: a pass over a subset of instances
+struct ChainSub {
+  PassSpec spec;
+  int pass = 0;          // index of the pass in its chain
+  std::vector<int> ids;  // member instance ids
+  int nt = kNT, grid = 0;
+  size_t lds = 0;
+  void* tab = nullptr;   // device PassArgs<T>[members]
+  int* seg = nullptr;    // device: block -> member, then member -> first block
+  int members = 0;
+};
+
+struct Chain {
+  int esize = 8;
+  std::vector<ChainSub> subs;
+  void release() {
+    for (ChainSub& s : subs) {
+      if (s.tab) (void)hipFree(s.tab);
+      if (s.seg) (void)hipFree(s.seg);
+    }
+    subs.clear();
+  }
+};
+
+template <typename T>
+int run_chain(const Chain& c, hipStream_t st) {
+  for (const ChainSub& s : c.subs) {
+    BSeg seg{s.seg, s.seg + s.grid};
+    if (s.nt == kNT) CHK((launch_spec<T, kNT>(s.spec, s.grid, s.lds, st, s.tab, seg)));
+    else CHK((launch_spec<T, kNTBig>(s.spec, s.grid, s.lds, st, s.tab, seg)));
+  }
+  return CSIPM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// the engine
+// ---------------------------------------------------------------------------
+constexpr int64_t kSlotAlign = 256;   // slot ranges start on 256-slot boundaries
+constexpr int kPartPerAlign = 16;     // partial-sum slots per 256 vector slots
+
+// FP64 control vectors (one block, pool length) and two-block FP64 vectors
+enum { V_B = 0, V_BETA, V_C, V_Z1, V_Z2, V_QB, V_S1, V_S2, V_DT1, V_DT2, V_DZ1, V_DZ2, V_L1D,
+       V_L2D, V_DGD, V_NV1 };
+enum { V2_RHSD = 0, V2_DD, V2_TMP, V_NV2 };
+// CG-precision vectors: one block (L1 or D, L2, P11 or P, P12, P22), two-block
+// (rhs, d) and the CG vectors r, z, p, q (two-block)
+enum { C_L1 = 0, C_L2, C_P11, C_P12, C_P22, C_NV1 };
+enum { C2_RHS = 0, C2_D, C2_R, C2_Z, C2_P, C2_Q, C_NV2 };
+
+// chains of the interior-point iteration
+enum { CALL_QB = 0, CALL_REF, CALL_CGA, CALL_CGR, CALL_MTZ, CALL_N };
+
+struct Engine {
+  int tc_size = 8;          // CG / operator precision of the CG
+  bool mixed = false;       // reduced system, FP32 CG, FP64 refinement
+  hipStream_t stream = nullptr;
+  int64_t cap = 0;          // pool slots
+  int maxinst = 0;
+  struct Inst {
+    csipm_op* h = nullptr;
+    int64_t off = 0, len = 0;
+    bool live = false;      // admitted and not yet retired
+    bool fresh = false;     // admitted, not yet initialised
+    double lam = 0;
+    const double* z0 = nullptr;
+    double* beta_out = nullptr;
+  };
+  std::vector<Inst> inst;
+  std::vector<std::pair<int64_t, int64_t>> freelist;   // (offset, length)
+  // device state
+  BInst* info = nullptr;
+  BFlags* flags = nullptr;
+  BFlags* flags_h = nullptr;
+  CgState* cg = nullptr;
+  CgState* cg_h = nullptr;
+  double* scal = nullptr;
+  double* part = nullptr;
+  uint32_t* valid = nullptr;
+  double* Wd = nullptr;      // V_NV1 * cap + V_NV2 * 2 cap doubles
+  void* Wc = nullptr;        // C_NV1 * cap + C_NV2 * 2 cap CG-precision elements
+  // launch tables of the current active set
+  std::vector<int> active;   // instance ids
+  int* seg_mem = nullptr;    // vector segment (block -> inst, inst -> first block) + ids
+  int vgrid = 0;
+  BSeg vseg{};
+  const int* ids = nullptr;
+  Chain chains[CALL_N];
+  bool tables_dirty = true;
+
+  double* W(int k) { return Wd + (size_t)k * cap; }
+  double* W2(int k) { return Wd + (size_t)V_NV1 * cap + (size_t)k * 2 * cap; }
+  template <typename T>
+  T* C(int k) { return (T*)Wc + (size_t)k * cap; }
+  template <typename T>
+  T* C2(int k) { return (T*)Wc + (size_t)C_NV1 * cap + (size_t)k * 2 * cap; }
+
+  void release_tables() {
+    for (Chain& c : chains) c.release();
+    if (seg_mem) (void)hipFree(seg_mem);
+    seg_mem = nullptr;
+  }
+  ~Engine() {
+    release_tables();
+    for (void* p : {(void*)info, (void*)flags, (void*)cg, (void*)scal, (void*)part, (void*)valid,
+                    (void*)Wd, Wc})
+      if (p) (void)hipFree(p);
+    if (flags_h) (void)hipHostFree(flags_h);
+    if (cg_h) (void)hipHostFree(cg_h);
+  }
+};
+
+int64_t slots_of(const csipm_op* h) { return (h->U + kSlotAlign - 1) / kSlotAlign * kSlotAlign; }
+
+// device bytes of the engine pool per slot
+size_t engine_bytes_per_slot(int tc_size) {
+  return sizeof(double) * (V_NV1 + 2 * V_NV2) + (size_t)tc_size * (C_NV1 + 2 * C_NV2) +
+         sizeof(double) * kPartPerAlign / kSlotAlign + 1;
+}
+
+int engine_create(Engine** out, int64_t cap_slots, int maxinst, int tc_size, bool mixed,
+                  hipStream_t st) {
+  Engine* E = new (std::nothrow) Engine();
+  if (!E) return fail(CSIPM_ERR_ALLOC, "out of host memory");
+  E->tc_size = tc_size;
+  E->mixed = mixed;
+  E->stream = st;
+  E->cap = (cap_slots + kSlotAlign - 1) / kSlotAlign * kSlotAlign;
+  E->maxinst = maxinst;
+  E->inst.resize((size_t)maxinst);
+  E->freelist.push_back({0, E->cap});
+  auto bad = [&](int rc) { delete E; return rc; };
+#define ECHK(expr)                                                                       \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      return bad(fail(CSIPM_ERR_ALLOC, "%s failed: %s", #expr, hipGetErrorString(e_))); \
+  } while (0)
+  ECHK(hipMalloc(&E->info, sizeof(BInst) * (size_t)maxinst));
+  ECHK(hipMalloc(&E->flags, sizeof(BFlags) * (size_t)maxinst));
+  ECHK(hipHostMalloc(&E->flags_h, sizeof(BFlags) * (size_t)maxinst, hipHostMallocDefault));
+  ECHK(hipMalloc(&E->cg, sizeof(CgState) * (size_t)maxinst));
+  ECHK(hipHostMalloc(&E->cg_h, sizeof(CgState) * (size_t)maxinst, hipHostMallocDefault));
+  ECHK(hipMalloc(&E->scal, sizeof(double) * S_NSLOTS * (size_t)maxinst));
+  ECHK(hipMalloc(&E->part, sizeof(double) * (size_t)(E->cap / kSlotAlign * kPartPerAlign)));
+  ECHK(hipMalloc(&E->valid, sizeof(uint32_t) * (size_t)(E->cap / 32)));
+  ECHK(hipMalloc(&E->Wd, sizeof(double) * (size_t)E->cap * (V_NV1 + 2 * V_NV2)));
+  ECHK(hipMalloc(&E->Wc, (size_t)tc_size * (size_t)E->cap * (C_NV1 + 2 * C_NV2)));
+#undef ECHK
+  *out = E;
+  return CSIPM_OK;
+}
+
+// first-fit slot allocation
+bool slots_alloc(Engine* E, int64_t len, int64_t* off) {
+  for (size_t i = 0; i < E->freelist.size(); ++i) {
+    auto& f = E->freelist[i];
+    if (f.second >= len) {
+      *off = f.first;
+      f.first += len;
+      f.second -= len;
+      if (f.second == 0) E->freelist.erase(E->freelist.begin() + (long)i);
+      return true;
+    }
+  }
+  return false;
+}
+
+void slots_free(Engine* E, int64_t off, int64_t len) {
+  E->freelist.push_back({off, len});
+  std::sort(E->freelist.begin(), E->freelist.end());
+  std::vector<std::pair<int64_t, int64_t>> merged;
+  for (auto& f : E->freelist) {
+    if (!merged.empty() && merged.back().first + merged.back().second == f.first)
+      merged.back().second += f.second;
+    else
+      merged.push_back(f);
+  }
+  E->freelist.swap(merged);
+}
+
+// vector-kernel blocks of an instance: 1024 slots per block, at most 2048
+int vblocks(int64_t n) {
+  int64_t b = (n + 1023) / 1024;
+  return (int)std::max<int64_t>(1, std::min<int64_t>(b, 2048));
+}
+
+// per-instance epilogue / source of a chain call
+template <typename T>
+int chain_call_args(Engine* E, int id, int call, int nb, const PassSpec& s, int pass_last,
+                    PassArgs<T>& a) {
+  Engine::Inst& in = E->inst[(size_t)id];
+  csipm_op* h = in.h;
+  const int64_t off = in.off, n = h->U;
+  Epi<T> e{};
+  const void* src = nullptr;
+  const T* real_in = nullptr;
+  const int* done = &E->flags[id].done;
+  T scale = (T)(1.0 / (double)h->N);
+  const int64_t o2 = 2 * off;
+  switch (call) {
+    case CALL_QB:
+      src = E->W(V_BETA) + off;
+      e.kind = 0;
+      e.out = (T*)(E->W(V_QB) + off);
+      break;
+    case CALL_REF:
+      src = E->W2(V2_DD) + o2;
+      e.kind = 2;
+      e.xb = (const T*)(E->W2(V2_DD) + o2);
+      e.L1 = (const T*)(E->W(V_DGD) + off);
+      e.yb = (T*)(E->W2(V2_TMP) + o2);
+      e.alpha = (T)-1;
+      e.beta = (T)1;
+      break;
+    case CALL_CGA:
+    case CALL_CGR: {
+      T* x = E->C2<T>(call == CALL_CGA ? C2_P : C2_D) + o2;
+      T* q = E->C2<T>(C2_Q) + o2;
+      src = x;
+      e.kind = nb == 2 ? 1 : 2;
+      e.xb = x;
+      e.xz = x + n;
+      e.L1 = E->C<T>(C_L1) + off;
+      e.L2 = E->C<T>(C_L2) + off;
+      e.yb = q;
+      e.yz = q + n;
+      e.alpha = (T)1;
+      e.beta = (T)0;
+      if (call == CALL_CGA) e.dot_part = E->part + off / kSlotAlign * kPartPerAlign;
+      done = &E->cg[id].done;
+      break;
+    }
+    case CALL_MTZ:
+      real_in = (const T*)in.z0;
+      scale = (T)(1.0 / std::sqrt((double)h->N));
+      e.kind = 0;
+      e.out = (T*)(E->W(V_B) + off);
+      break;
+    default:
+      return fail(CSIPM_ERR_ARG, "bad chain call");
+  }
+  (void)pass_last;
+  if (s.kind == PK_LINES) a = lines_args<T>(h, s.axis, src, e, done);
+  else a = dim1_args<T>(h, src, real_in, nullptr, e, scale, done);
+  return CSIPM_OK;
+}
+
+template <typename T>
+int build_chain(Engine* E, const std::vector<int>& ids, int call, int nb, Chain& out) {
+  out.release();
+  out.esize = sizeof(T);
+  for (int dim = 1; dim <= 3; ++dim) {
+    std::vector<int> mem;
+    for (int id : ids)
+      if (E->inst[(size_t)id].h->dim == dim) mem.push_back(id);
+    if (mem.empty()) continue;
+    PassSpec specs[8];
+    const int np = chain_specs(dim, call == CALL_MTZ ? CH_MTZ : CH_MTM, specs);
+    for (int p = 0; p < np; ++p) {
+      const PassSpec& s = specs[p];
+      for (int nt : {kNT, kNTBig}) {
+        std::vector<PassArgs<T>> tab;
+        std::vector<int> grids;
+        size_t lds = 0;
+        for (int id : mem) {
+          csipm_op* h = E->inst[(size_t)id].h;
+          const PrecState& ps = h->ps<T>();
+          const int want = s.kind == PK_LINES ? kNT : ps.d1.nt;
+          if (want != nt) continue;
+          PassArgs<T> a{};
+          CHK(chain_call_args<T>(E, id, call, nb, s, p == np - 1, a));
+          tab.push_back(a);
+          if (s.kind == PK_LINES) {
+            grids.push_back(ps.lines[s.axis].grid);
+            lds = std::max(lds, ps.lines[s.axis].lds);
+          } else {
+            grids.push_back(ps.d1.grid);
+            lds = std::max(lds, ps.d1.lds);
+          }
+        }
+        if (tab.empty()) continue;
+        ChainSub sub;
+        sub.spec = s;
+        sub.pass = p;
+        for (int id : mem) {
+          csipm_op* hm = E->inst[(size_t)id].h;
+          const int want = s.kind == PK_LINES ? kNT : hm->ps<T>().d1.nt;
+          if (want == nt) sub.ids.push_back(id);
+        }
+        sub.nt = nt;
+        sub.lds = lds;
+        sub.members = (int)tab.size();
+        std::vector<int> segv;
+        std::vector<int> first;
+        int tot = 0;
+        for (size_t m = 0; m < tab.size(); ++m) {
+          first.push_back(tot);
+          for (int b = 0; b < grids[m]; ++b) segv.push_back((int)m);
+          tot += grids[m];
+        }
+        first.push_back(tot);
+        sub.grid = tot;
+        segv.insert(segv.end(), first.begin(), first.end());
+        HIPCHK(hipMalloc(&sub.tab, sizeof(PassArgs<T>) * tab.size()));
+        HIPCHK(hipMalloc(&sub.seg, sizeof(int) * segv.size()));
+        HIPCHK(hipMemcpyAsync(sub.tab, tab.data(), sizeof(PassArgs<T>) * tab.size(),
+                              hipMemcpyHostToDevice, E->stream));
+        HIPCHK(hipMemcpyAsync(sub.seg, segv.data(), sizeof(int) * segv.size(),
+                              hipMemcpyHostToDevice, E->stream));
+        out.subs.push_back(sub);
+      }
+    }
+  }
+  // the host vectors above are freed on return: finish the copies first
+  HIPCHK(hipStreamSynchronize(E->stream));
+  return CSIPM_OK;
+}
+
+// final-pass partial count of the CG apply of instance h (precision T)
+template <typename T>
+int cg_final_grid(csipm_op* h) {
+  if (h->dim == 1) return h->ps<T>().d1.grid;
+  return h->ps<T>().lines[h->dim - 1].grid;
+}
+
+// vector segment + id list of `ids` into one device allocation
+int upload_vseg(Engine* E, const std::vector<int>& ids, int** mem, BSeg* seg, const int** idp,
+                int* grid) {
+  std::vector<int> blk, first(E->maxinst + 1, 0);
+  int tot = 0;
+  // first[] is indexed by instance id: ids not in the set get empty ranges
+  std::vector<int> nb(E->maxinst, 0);
+  for (int id : ids) nb[(size_t)id] = vblocks(E->inst[(size_t)id].h->U);
+  for (int id = 0; id < E->maxinst; ++id) {
+    first[(size_t)id] = tot;
+    for (int b = 0; b < nb[(size_t)id]; ++b) blk.push_back(id);
+    tot += nb[(size_t)id];
+  }
+  first[(size_t)E->maxinst] = tot;
+  std::vector<int> all(blk);
+  all.insert(all.end(), first.begin(), first.end());
+  all.insert(all.end(), ids.begin(), ids.end());
+  if (*mem) (void)hipFree(*mem);
+  *mem = nullptr;
+  HIPCHK(hipMalloc(mem, sizeof(int) * std::max<size_t>(all.size(), 1)));
+  HIPCHK(hipMemcpyAsync(*mem, all.data(), sizeof(int) * all.size(), hipMemcpyHostToDevice,
+                        E->stream));
+  HIPCHK(hipStreamSynchronize(E->stream));
+  seg->inst = *mem;
+  seg->first = *mem + blk.size();
+  *idp = *mem + blk.size() + first.size();
+  *grid = tot;
+  return CSIPM_OK;
+}
+
+// launch tables of the active set: the vector segment and the chains of the
+// interior-point iteration (ipm = false: only the CG apply, for a plain CG)
+template <typename TC>
+int engine_tables(Engine* E, int nb, bool ipm) {
+  if (!E->tables_dirty) return CSIPM_OK;
+  E->release_tables();
+  CHK(upload_vseg(E, E->active, &E->seg_mem, &E->vseg, &E->ids, &E->vgrid));
+  CHK(build_chain<TC>(E, E->active, CALL_CGA, nb, E->chains[CALL_CGA]));
+  if (ipm) {
+    CHK(build_chain<double>(E, E->active, CALL_QB, nb, E->chains[CALL_QB]));
+    if (E->mixed) {
+      CHK(build_chain<double>(E, E->active, CALL_REF, nb, E->chains[CALL_REF]));
+      CHK(build_chain<TC>(E, E->active, CALL_CGR, nb, E->chains[CALL_CGR]));
+    }
+  }
+  E->tables_dirty = false;
+  return CSIPM_OK;
+}
+
+// admit instance h (problem data z0 (device, N reals), lambda; the solution
+// goes to beta_out (device, N doubles); CG iteration cap per solve, <= 0:
+// 4N).  Returns the id through *id.
+int engine_admit(Engine* E, csipm_op* h, const double* z0, double lam, double* beta_out,
+                 int64_t cg_itmax, int* id) {
+  int k = -1;
+  for (int i = 0; i < E->maxinst; ++i)
+    if (!E->inst[(size_t)i].h) {
+      k = i;
+      break;
+    }
+  if (k < 0) return fail(CSIPM_ERR_ALLOC, "engine full (%d instances)", E->maxinst);
+  const int64_t len = slots_of(h);
+  int64_t off = 0;
+  if (!slots_alloc(E, len, &off))
+    return fail(CSIPM_ERR_ALLOC, "no free slot range of %lld", (long long)len);
+  Engine::Inst& in = E->inst[(size_t)k];
+  in.h = h;
+  in.off = off;
+  in.len = len;
+  in.live = true;
+  in.fresh = z0 != nullptr;   // no problem data: the instance only runs CG
+  in.lam = lam;
+  in.z0 = z0;
+  in.beta_out = beta_out;
+  // partial-sum capacity: 8 quantities per vector block, the CG apply's final pass
+  const int64_t pcap = len / kSlotAlign * kPartPerAlign;
+  const int gcg = E->tc_size == 8 ? cg_final_grid<double>(h) : cg_final_grid<float>(h);
+  if (8 * (int64_t)vblocks(h->U) > pcap || gcg > pcap)
+    return fail(CSIPM_ERR_UNSUPPORTED, "partial sums of instance exceed its share");
+  BInst bi{};
+  bi.off = off;
+  bi.n = h->U;
+  bi.N = h->N;
+  bi.lam = lam;
+  bi.pbase = off / kSlotAlign * kPartPerAlign;
+  bi.gcg = gcg;
+  bi.itmax = (int)std::min<int64_t>(cg_itmax > 0 ? cg_itmax : 4 * h->N, 1 << 30);
+  BFlags fl{};
+  HIPCHK(hipMemcpyAsync(E->info + k, &bi, sizeof(BInst), hipMemcpyHostToDevice, E->stream));
+  HIPCHK(hipMemcpyAsync(E->flags + k, &fl, sizeof(BFlags), hipMemcpyHostToDevice, E->stream));
+  HIPCHK(hipMemcpyAsync(E->valid + off / 32, h->valid, sizeof(uint32_t) * (size_t)((h->U + 31) / 32),
+                        hipMemcpyDeviceToDevice, E->stream));
+  HIPCHK(hipStreamSynchronize(E->stream));
+  E->active.push_back(k);
+  E->tables_dirty = true;
+  *id = k;
+  return CSIPM_OK;
+}
+
+void engine_drop(Engine* E, int id) {
+  Engine::Inst& in = E->inst[(size_t)id];
+  slots_free(E, in.off, in.len);
+  in = Engine::Inst{};
+  E->active.erase(std::remove(E->active.begin(), E->active.end(), id), E->active.end());
+  E->tables_dirty = true;
+}
+
+int finalize(Engine* E, const BSeg& vseg, const int* ids, int nids, std::initializer_list<int> ops,
+             int mode, int first, double tau, double tol, int max_iter) {
+  RedSpec spec{};
+  spec.nq = 0;
+  for (int op : ops) spec.op[spec.nq++] = op;
+  hipLaunchKernelGGL(bk_ipm_finalize, dim3(nids), dim3(kNT), 0, E->stream, vseg, ids, E->info,
+                     E->flags, E->part, spec, E->scal, mode, first, tau, tol, max_iter);
+  HIPCHK(hipGetLastError());
+  return CSIPM_OK;
+}
+
+// initialise the fresh instances: b = M^T z0 (u order), ||b||_inf, the starting point
+int engine_init_fresh(Engine* E) {
+  std::vector<int> fresh;
+  for (int id : E->active)
+    if (E->inst[(size_t)id].fresh) fresh.push_back(id);
+  if (fresh.empty()) return CSIPM_OK;
+  Chain mtz;
+  int* mem = nullptr;
+  BSeg seg{};
+  const int* ids = nullptr;
+  int grid = 0;
+  int rc = CSIPM_OK;
+  do {
+    if ((rc = build_chain<double>(E, fresh, CALL_MTZ, 2, mtz)) != CSIPM_OK) break;
+    if ((rc = upload_vseg(E, fresh, &mem, &seg, &ids, &grid)) != CSIPM_OK) break;
+    if ((rc = run_chain<double>(mtz, E->stream)) != CSIPM_OK) break;
+    hipLaunchKernelGGL(bk_absmax, dim3(grid), dim3(kNT), 0, E->stream, seg, E->info, E->flags,
+                       E->W(V_B), E->part);
+    if ((rc = finalize(E, seg, ids, (int)fresh.size(), {1}, 0, S_BNORM, 0, 0, 0)) != CSIPM_OK)
+      break;
+    hipLaunchKernelGGL(bk_ipm_init, dim3(grid), dim3(kNT), 0, E->stream, seg, E->info, E->flags,
+                       E->W(V_BETA), E->W(V_C), E->W(V_Z1), E->W(V_Z2), E->scal, E->valid);
+    if (hipGetLastError() != hipSuccess) {
+      rc = fail(CSIPM_ERR_HIP, "init launch failed");
+      break;
+    }
+    if (hipStreamSynchronize(E->stream) != hipSuccess) {
+      rc = fail(CSIPM_ERR_HIP, "init failed");
+      break;
+    }
+  } while (false);
+  mtz.release();
+  if (mem) (void)hipFree(mem);
+  if (rc) return rc;
+  for (int id : fresh) E->inst[(size_t)id].fresh = false;
+  return CSIPM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// batched preconditioned CG on the engine's CG vectors: b = C2_RHS,
+// x = C2_D, diagonals C_L1.. (nb = 2: condensed K with the block
+// preconditioner; nb = 1: reduced system with D in C_L1, P in C_P11).
+// reliable > 0: true residual and restart every `reliable` iterations.
+// ---------------------------------------------------------------------------
+template <typename TC>
+int engine_cg(Engine* E, int nb, double atol, double rtol, int reliable) {
+  const int na = (int)E->active.size();
+  TC *b = E->C2<TC>(C2_RHS), *x = E->C2<TC>(C2_D), *r = E->C2<TC>(C2_R);
+  TC *z = E->C2<TC>(C2_Z), *p = E->C2<TC>(C2_P), *q = E->C2<TC>(C2_Q);
+  TC *P11 = E->C<TC>(C_P11), *P12 = E->C<TC>(C_P12), *P22 = E->C<TC>(C_P22);
+  hipStream_t st = E->stream;
+  hipLaunchKernelGGL((bk_cg_init<TC>), dim3(E->vgrid), dim3(kNT), 0, st, E->vseg, E->info,
+                     E->flags, b, x, r, z, p, P11, P12, P22, nb, E->part);
+  hipLaunchKernelGGL(bk_cg_start, dim3(na), dim3(kNT), 0, st, E->vseg, E->ids, E->info, E->flags,
+                     E->cg, E->part, atol, rtol);
+  HIPCHK(hipGetLastError());
+  const int check_every = 4;
+  for (int64_t it = 0;; ++it) {
+    if (it % check_every == 0) {
+      HIPCHK(hipMemcpyAsync(E->cg_h, E->cg, sizeof(CgState) * (size_t)E->maxinst,
+                            hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      bool all = true;
+      for (int id : E->active) all = all && E->cg_h[id].done;
+      if (all) break;
+    }
+    CHK(run_chain<TC>(E->chains[CALL_CGA], st));
+    hipLaunchKernelGGL(bk_cg_alpha, dim3(na), dim3(kNT), 0, st, E->ids, E->info, E->cg, E->part);
+    hipLaunchKernelGGL((bk_cg_update<TC>), dim3(E->vgrid), dim3(kNT), 0, st, E->vseg, E->info,
+                       E->cg, x, r, z, p, q, P11, P12, P22, nb, E->part);
+    hipLaunchKernelGGL(bk_cg_gamma, dim3(na), dim3(kNT), 0, st, E->vseg, E->ids, E->info, E->cg,
+                       E->part);
+    if (reliable > 0 && (it + 1) % reliable == 0) {
+      CHK(run_chain<TC>(E->chains[CALL_CGR], st));
+      hipLaunchKernelGGL((bk_cg_residual<TC>), dim3(E->vgrid), dim3(kNT), 0, st, E->vseg,
+                         E->info, E->cg, b, q, r, z, P11, P12, P22, nb, E->part);
+      hipLaunchKernelGGL(bk_cg_regamma, dim3(na), dim3(kNT), 0, st, E->vseg, E->ids, E->info,
+                         E->cg, E->part);
+    }
+    hipLaunchKernelGGL((bk_cg_pupdate<TC>), dim3(E->vgrid), dim3(kNT), 0, st, E->vseg, E->info,
+                       E->cg, p, z, nb);
+    HIPCHK(hipGetLastError());
+  }
+  hipLaunchKernelGGL(bk_cg_account, dim3((na + 255) / 256), dim3(256), 0, st, na, E->ids, E->cg,
+                     E->flags);
+  HIPCHK(hipGetLastError());
+  return CSIPM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// one Newton solve.  FP64: CG on the condensed system (the reference's K),
+// direction in C2_D.  Mixed: the c-block is eliminated in FP64
+// (dc = (h_c - L2 db) / L1 leaves (M^T M + D) db = h_b - (L2/L1) h_c with
+// D = L1 - L2^2/L1 = 4 S1 S2 / (S1 + S2) > 0), the reduced system is solved
+// by FP32 CG with the diagonal preconditioner 1/(1 + D), then one FP64
+// refinement step; direction in V2_DD (FP64).  In FP32 the condensed K itself
+// is unusable near the solution: L1 and |L2| agree to more than 24 bits there.
+// ---------------------------------------------------------------------------
+template <typename TC>
+int engine_newton(Engine* E, double rtol, int reliable) {
+  if (!E->mixed) return engine_cg<TC>(E, 2, 0.0, rtol, 0);
+  hipStream_t st = E->stream;
+  const BSeg& sg = E->vseg;
+  CHK(engine_cg<TC>(E, 1, 0.0, rtol, reliable));
+  hipLaunchKernelGGL((bk_convert<TC, double>), dim3(E->vgrid), dim3(kNT), 0, st, sg, E->info,
+                     E->flags, E->C2<TC>(C2_D), E->W2(V2_DD), 1);
+  hipLaunchKernelGGL((bk_convert<double, double>), dim3(E->vgrid), dim3(kNT), 0, st, sg, E->info,
+                     E->flags, E->W2(V2_RHSD), E->W2(V2_TMP), 1);
+  CHK(run_chain<double>(E->chains[CALL_REF], st));
+  hipLaunchKernelGGL((bk_convert<double, TC>), dim3(E->vgrid), dim3(kNT), 0, st, sg, E->info,
+                     E->flags, E->W2(V2_TMP), E->C2<TC>(C2_RHS), 1);
+  HIPCHK(hipGetLastError());
+  CHK(engine_cg<TC>(E, 1, 0.0, rtol, reliable));
+  hipLaunchKernelGGL((bk_widen_add<TC>), dim3(E->vgrid), dim3(kNT), 0, st, sg, E->info, E->flags,
+                     E->W2(V2_DD), E->C2<TC>(C2_D));
+  hipLaunchKernelGGL(bk_red_back, dim3(E->vgrid), dim3(kNT), 0, st, sg, E->info, E->flags,
+                     E->W2(V2_DD), E->W2(V2_RHSD), E->W(V_L1D), E->W(V_L2D), E->valid);
+  HIPCHK(hipGetLastError());
+  return CSIPM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// One interior-point iteration of every active instance: residuals and the
+// stopping test, then (for those still running) a Mehrotra predictor-
+// corrector step.  *running = number of instances still iterating.
+// ---------------------------------------------------------------------------
+template <typename TC>
+int engine_step(Engine* E, double tol, int max_iter, double cg_rtol, int* running) {
+  const int nb = E->mixed ? 1 : 2;
+  CHK(engine_init_fresh(E));
+  CHK(engine_tables<TC>(E, nb, true));
+  const int na = (int)E->active.size();
+  *running = 0;
+  if (na == 0) return CSIPM_OK;
+  hipStream_t st = E->stream;
+  const BSeg& sg = E->vseg;
+  const int g = E->vgrid;
+  const double tau = 0.995;
+  const double rtol = E->mixed ? std::max(cg_rtol, 1e-5) : cg_rtol;
+  const int reliable = E->mixed ? 25 : 0;
+  double *b = E->W(V_B), *beta = E->W(V_BETA), *c = E->W(V_C), *z1 = E->W(V_Z1);
+  double *z2 = E->W(V_Z2), *Qb = E->W(V_QB), *s1 = E->W(V_S1), *s2 = E->W(V_S2);
+  double *dt1 = E->W(V_DT1), *dt2 = E->W(V_DT2), *dz1 = E->W(V_DZ1), *dz2 = E->W(V_DZ2);
+  CHK(run_chain<double>(E->chains[CALL_QB], st));
+  hipLaunchKernelGGL(bk_ipm_resid, dim3(g), dim3(kNT), 0, st, sg, E->info, E->flags, beta, c, z1,
+                     z2, Qb, b, E->part, E->valid);
+  CHK(finalize(E, sg, E->ids, na, {1, 1, 0, 0, 0, 0}, 0, S_RB, tau, tol, max_iter));
+  HIPCHK(hipMemcpyAsync(E->flags_h, E->flags, sizeof(BFlags) * (size_t)E->maxinst,
+                        hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  int run = 0;
+  for (int id : E->active) run += E->flags_h[id].done ? 0 : 1;
+  *running = run;
+  if (run == 0) return CSIPM_OK;
+  TC* rhs = E->C2<TC>(C2_RHS);
+  double* rhsd = E->mixed ? E->W2(V2_RHSD) : nullptr;
+  if (E->mixed)
+    hipLaunchKernelGGL((bk_ipm_prep_red<TC>), dim3(g), dim3(kNT), 0, st, sg, E->info, E->flags,
+                       beta, c, z1, z2, Qb, b, s1, s2, E->C<TC>(C_L1), E->C<TC>(C_P11), rhs,
+                       E->W(V_L1D), E->W(V_L2D), E->W(V_DGD), rhsd, E->valid);
+  else
+    hipLaunchKernelGGL((bk_ipm_prep<TC>), dim3(g), dim3(kNT), 0, st, sg, E->info, E->flags, beta,
+                       c, z1, z2, Qb, b, s1, s2, E->C<TC>(C_L1), E->C<TC>(C_L2),
+                       E->C<TC>(C_P11), E->C<TC>(C_P12), E->C<TC>(C_P22), rhs, E->valid);
+  HIPCHK(hipGetLastError());
+  // predictor
+  CHK(engine_newton<TC>(E, rtol, reliable));
+  auto dir = [&](int corr) -> int {
+    if (E->mixed)
+      hipLaunchKernelGGL((bk_ipm_dir<double>), dim3(g), dim3(kNT), 0, st, sg, E->info, E->flags,
+                         beta, c, z1, z2, s1, s2, E->W2(V2_DD), E->scal, corr, dt1, dt2, dz1, dz2,
+                         E->part, E->valid);
+    else
+      hipLaunchKernelGGL((bk_ipm_dir<TC>), dim3(g), dim3(kNT), 0, st, sg, E->info, E->flags, beta,
+                         c, z1, z2, s1, s2, E->C2<TC>(C2_D), E->scal, corr, dt1, dt2, dz1, dz2,
+                         E->part, E->valid);
+    HIPCHK(hipGetLastError());
+    return CSIPM_OK;
+  };
+  CHK(dir(0));
+  CHK(finalize(E, sg, E->ids, na, {2, 2}, 1, 0, tau, tol, max_iter));
+  hipLaunchKernelGGL(bk_ipm_muaff, dim3(g), dim3(kNT), 0, st, sg, E->info, E->flags, beta, c, z1,
+                     z2, dt1, dt2, dz1, dz2, E->scal, E->part, E->valid);
+  CHK(finalize(E, sg, E->ids, na, {0}, 2, 0, tau, tol, max_iter));
+  // corrector
+  hipLaunchKernelGGL((bk_ipm_rhs_corr<TC>), dim3(g), dim3(kNT), 0, st, sg, E->info, E->flags,
+                     beta, c, z1, z2, Qb, b, dt1, dt2, dz1, dz2, E->scal, rhs, rhsd, E->W(V_L1D),
+                     E->W(V_L2D), E->valid);
+  HIPCHK(hipGetLastError());
+  CHK(engine_newton<TC>(E, rtol, reliable));
+  CHK(dir(1));
+  CHK(finalize(E, sg, E->ids, na, {2, 2}, 3, 0, tau, tol, max_iter));
+  if (E->mixed)
+    hipLaunchKernelGGL((bk_ipm_update<double>), dim3(g), dim3(kNT), 0, st, sg, E->info, E->flags,
+                       beta, c, z1, z2, E->W2(V2_DD), dz1, dz2, E->scal, E->valid);
+  else
+    hipLaunchKernelGGL((bk_ipm_update<TC>), dim3(g), dim3(kNT), 0, st, sg, E->info, E->flags,
+                       beta, c, z1, z2, E->C2<TC>(C2_D), dz1, dz2, E->scal, E->valid);
+  HIPCHK(hipGetLastError());
+  return CSIPM_OK;
+}
+
+// results of a finished instance: beta (reference layout) into its beta_out,
+// info[8] = [objective, IPM iterations, CG iterations, status, dual residual,
+// mu, 0, 0]; the instance is retired and its slots freed
+int engine_harvest(Engine* E, int id, double* info) {
+  Engine::Inst& in = E->inst[(size_t)id];
+  CHK(from_u<double>(in.h, E->W(V_BETA) + in.off, in.beta_out, E->stream));
+  HIPCHK(hipMemcpyAsync(E->flags_h + id, E->flags + id, sizeof(BFlags), hipMemcpyDeviceToHost,
+                        E->stream));
+  HIPCHK(hipStreamSynchronize(E->stream));
+  const BFlags& f = E->flags_h[id];
+  if (info) {
+    info[0] = f.f;
+    info[1] = f.iters;
+    info[2] = f.cgits;
+    info[3] = f.status;
+    info[4] = f.dual;
+    info[5] = f.mu;
+    info[6] = 0;
+    info[7] = 0;
+  }
+  engine_drop(E, id);
+  return CSIPM_OK;
+}
+
+template <typename TC>
+int engine_run(Engine* E, double tol, int max_iter, double cg_rtol) {
+  for (;;) {
+    int running = 0;
+    CHK(engine_step<TC>(E, tol, max_iter, cg_rtol, &running));
+    if (running == 0) return CSIPM_OK;
+  }
+}
+
+// per-sub timing of the CG apply chain of the active set (p = 0 -> q), HIP
+// events around every launch, averaged over nrep applies; bytes: the
+// algorithmic HBM bytes of each launch (sum over its instances)
+template <typename TC>
+int engine_apply_timed(Engine* E, int64_t nrep, double* ms, double* bytes, std::string* names,
+                       int cap, int* nsub) {
+  const int nb = E->mixed ? 1 : 2;
+  CHK(engine_init_fresh(E));
+  CHK(engine_tables<TC>(E, nb, true));
+  const Chain& c = E->chains[CALL_CGA];
+  const int ns = (int)c.subs.size();
+  if (ns > cap) return fail(CSIPM_ERR_ARG, "%d launches exceed the output arrays", ns);
+  HIPCHK(hipMemsetAsync(E->C2<TC>(C2_P), 0, sizeof(TC) * 2 * (size_t)E->cap, E->stream));
+  // every instance runs: mark the CG states not done
+  std::vector<CgState> live((size_t)E->maxinst);
+  for (auto& cs : live) cs = CgState{};
+  HIPCHK(hipMemcpyAsync(E->cg, live.data(), sizeof(CgState) * live.size(), hipMemcpyHostToDevice,
+                        E->stream));
+  std::vector<hipEvent_t> ev((size_t)ns + 1);
+  for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+  std::vector<double> acc((size_t)ns, 0.0);
+  for (int64_t r = 0; r < nrep; ++r) {
+    HIPCHK(hipEventRecord(ev[0], E->stream));
+    for (int k = 0; k < ns; ++k) {
+      const ChainSub& sb = c.subs[(size_t)k];
+      BSeg seg{sb.seg, sb.seg + sb.grid};
+      if (sb.nt == kNT) CHK((launch_spec<TC, kNT>(sb.spec, sb.grid, sb.lds, E->stream, sb.tab, seg)));
+      else CHK((launch_spec<TC, kNTBig>(sb.spec, sb.grid, sb.lds, E->stream, sb.tab, seg)));
+      HIPCHK(hipEventRecord(ev[(size_t)k + 1], E->stream));
+    }
+    HIPCHK(hipEventSynchronize(ev[(size_t)ns]));
+    for (int k = 0; k < ns; ++k) {
+      float t = 0;
+      HIPCHK(hipEventElapsedTime(&t, ev[(size_t)k], ev[(size_t)k + 1]));
+      acc[(size_t)k] += t;
+    }
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  const int layout = E->mixed ? 2 : 1;
+  for (int k = 0; k < ns; ++k) {
+    const ChainSub& sb = c.subs[(size_t)k];
+    double b = 0;
+    const char* nm[8];
+    double by[8];
+    for (int id : sb.ids) {
+      kkt_pass_info(E->inst[(size_t)id].h, layout, nm, by);
+      b += by[sb.pass];
+    }
+    kkt_pass_info(E->inst[(size_t)sb.ids[0]].h, layout, nm, by);
+    ms[k] = acc[(size_t)k] / (double)nrep;
+    bytes[k] = b;
+    names[k] = std::to_string(sb.spec.D) + "d:" + nm[sb.pass];
+  }
+  *nsub = ns;
+  return CSIPM_OK;
+}

@@ -272,9 +272,9 @@ __device__ __forceinline__ double scatter_elem(const Epi<T>& e, int k1, int k2, 
 }
 
 // per-block partial of a dot product (deterministic; finalised by a later kernel)
-__device__ __forceinline__ void store_partial(double* part, double acc) {
+__device__ __forceinline__ void store_partial(double* part, double acc, int idx) {
   const double s = block_reduce(acc, OpSum());
-  if (threadIdx.x == 0 && part) part[blockIdx.x] = s;
+  if (threadIdx.x == 0 && part) part[idx] = s;
 }
 
 // ===========================================================================
@@ -314,7 +314,57 @@ struct PassArgs {
   const uint32_t* miss;  // bit set = missing sample (nullable)
   T scale;
   const int* done;
+  // line transform: length len; bs = 1 selects Bluestein's chirp-z form, in
+  // which case plan is the power-of-two convolution FFT of length M >= 2 len - 1
+  int len;
+  int bs;
+  const cpx<T>* chirp;  // w_n = exp(-i pi n^2 / len), n < len
+  const cpx<T>* bfw;    // FFT_M(b) / M of the forward kernel b = conj(w) (wrapped)
+  const cpx<T>* binv;   // the same for the inverse kernel b = w
 };
+
+// DFT of length a.len of the 2^lg lines in LDS ([element][line], stride ps),
+// in place.  Direct: the Stockham plan.  Bluestein (lengths with a prime
+// factor above 17): X_k = w_k sum_n (x_n w_n) conj(w_{k-n}) (forward; the
+// inverse conjugates w), the convolution by FFT_M, pointwise product with the
+// precomputed kernel spectrum, inverse FFT_M.  LDS must hold M rows.
+template <typename T, int E, int NT, bool FWD>
+__device__ void line_dft(cpx<T>* lds, const PassArgs<T>& a, int lg, int ps) {
+  if (!a.bs) {
+    lds_fft<T, E, NT, FWD>(lds, a.plan, lg, ps, a.tw);
+    return;
+  }
+  const int L = a.len, M = a.plan.L, nl = 1 << lg;
+  for (int idx = threadIdx.x; idx < (M << lg); idx += NT) {
+    const int line = idx & (nl - 1), e = idx >> lg;
+    cpx<T>* q = lds + e * ps + line;
+    if (e < L) {
+      cpx<T> w = a.chirp[e];
+      if (!FWD) w.y = -w.y;
+      *q = cmul(*q, w);
+    } else {
+      *q = {(T)0, (T)0};
+    }
+  }
+  __syncthreads();
+  lds_fft<T, E, NT, true>(lds, a.plan, lg, ps, a.tw);
+  const cpx<T>* B = FWD ? a.bfw : a.binv;
+  for (int idx = threadIdx.x; idx < (M << lg); idx += NT) {
+    const int line = idx & (nl - 1), e = idx >> lg;
+    cpx<T>* q = lds + e * ps + line;
+    *q = cmul(*q, B[e]);
+  }
+  __syncthreads();
+  lds_fft<T, E, NT, false>(lds, a.plan, lg, ps, a.tw);
+  for (int idx = threadIdx.x; idx < (L << lg); idx += NT) {
+    const int line = idx & (nl - 1), e = idx >> lg;
+    cpx<T>* q = lds + e * ps + line;
+    cpx<T> w = a.chirp[e];
+    if (!FWD) w.y = -w.y;
+    *q = cmul(*q, w);
+  }
+  __syncthreads();
+}
 
 __device__ __forceinline__ int udiv(int a, int d, uint32_t m) { return d == 1 ? a : fdiv(a, m); }
 
@@ -332,15 +382,16 @@ __device__ __forceinline__ double spec_store(const PassArgs<T>& a, int k1, int k
   return epi_apply_u(a.epi, el, v_to_u<D, T>(v, k1, k2, k3, a.g));
 }
 
+// one workgroup's tile of a line pass (tile = its index within the pass)
 template <typename T, int NT, int E, int D, int SRC, int DST, bool FWD>
-__global__ __launch_bounds__(NT) void k_lines(PassArgs<T> a) {
+__device__ __forceinline__ void lines_body(const PassArgs<T>& a, int tile) {
   extern __shared__ unsigned char smem_raw[];
   cpx<T>* lds = reinterpret_cast<cpx<T>*>(smem_raw);
   if (a.done && *a.done) return;
-  const int L = a.plan.L;
+  const int L = a.len;
   const int lg = a.lg, nl = 1 << lg;
-  const int tile1 = blockIdx.x % a.ntile1;
-  const int o = blockIdx.x / a.ntile1;
+  const int tile1 = tile % a.ntile1;
+  const int o = tile / a.ntile1;
   const int k10 = tile1 << lg;
   const int nv = min(nl, a.H1 - k10);
   const int64_t s2 = a.H1, s3 = (int64_t)a.H1 * a.g.n[1];
@@ -363,7 +414,7 @@ __global__ __launch_bounds__(NT) void k_lines(PassArgs<T> a) {
     lds[idx] = val;
   }
   __syncthreads();
-  lds_fft<T, E, NT, FWD>(lds, a.plan, lg, nl, a.tw);
+  line_dft<T, E, NT, FWD>(lds, a, lg, nl);
   double acc = 0.0;
   for (int idx = threadIdx.x; idx < total; idx += NT) {
     const int line = idx & (nl - 1);
@@ -378,7 +429,24 @@ __global__ __launch_bounds__(NT) void k_lines(PassArgs<T> a) {
       acc += spec_store<D, T, DST>(a, k10 + line, k2, k3, val);
     }
   }
-  if (DST != DST_BUF && a.epi.dot_part) store_partial(a.epi.dot_part, acc);
+  if (DST != DST_BUF && a.epi.dot_part) store_partial(a.epi.dot_part, acc, tile);
+}
+
+template <typename T, int NT, int E, int D, int SRC, int DST, bool FWD>
+__global__ __launch_bounds__(NT) void k_lines(PassArgs<T> a) {
+  lines_body<T, NT, E, D, SRC, DST, FWD>(a, blockIdx.x);
+}
+
+// block -> (instance, tile) map of a launch over several instances
+struct BSeg {
+  const int* inst;   // instance of each block
+  const int* first;  // first block of each instance (n + 1 entries)
+};
+
+template <typename T, int NT, int E, int D, int SRC, int DST, bool FWD>
+__global__ __launch_bounds__(NT) void kb_lines(const PassArgs<T>* __restrict__ tab, BSeg seg) {
+  const int i = seg.inst[blockIdx.x];
+  lines_body<T, NT, E, D, SRC, DST, FWD>(tab[i], blockIdx.x - seg.first[i]);
 }
 
 __device__ __forceinline__ bool miss_bit(const uint32_t* m, int64_t s) {
@@ -386,13 +454,13 @@ __device__ __forceinline__ bool miss_bit(const uint32_t* m, int64_t s) {
 }
 
 template <typename T, int NT, int E, int D, int SRC, int DST, int MODE>
-__global__ __launch_bounds__(NT) void k_dim1(PassArgs<T> a) {
+__device__ __forceinline__ void dim1_body(const PassArgs<T>& a, int tile) {
   extern __shared__ unsigned char smem_raw[];
   cpx<T>* lds = reinterpret_cast<cpx<T>*>(smem_raw);
   if (a.done && *a.done) return;
-  const int N1 = a.plan.L, H1 = a.H1;
+  const int N1 = a.len, H1 = a.H1;
   const int lg = a.lg, np = 1 << lg, ps = a.ps;
-  const int p0 = blockIdx.x << lg;
+  const int p0 = tile << lg;
   const bool even = (N1 & 1) == 0;
   if (MODE != MODE_R2C) {
     const int tot = H1 << lg;
@@ -417,7 +485,7 @@ __global__ __launch_bounds__(NT) void k_dim1(PassArgs<T> a) {
       if (k > 0 && N1 - k != k) lds[(N1 - k) * ps + p] = {A.x + B.y, B.x - A.y};
     }
     __syncthreads();
-    lds_fft<T, E, NT, false>(lds, a.plan, lg, ps, a.tw);
+    line_dft<T, E, NT, false>(lds, a, lg, ps);
     const int totr = N1 << lg;
     for (int idx = threadIdx.x; idx < totr; idx += NT) {
       const int p = idx & (np - 1);
@@ -449,7 +517,7 @@ __global__ __launch_bounds__(NT) void k_dim1(PassArgs<T> a) {
     }
     __syncthreads();
   }
-  lds_fft<T, E, NT, true>(lds, a.plan, lg, ps, a.tw);
+  line_dft<T, E, NT, true>(lds, a, lg, ps);
   double acc = 0.0;
   const int tot = H1 << lg;
   const T half = (T)0.5;
@@ -469,7 +537,18 @@ __global__ __launch_bounds__(NT) void k_dim1(PassArgs<T> a) {
       acc += spec_store<D, T, DST>(a, k, 0, 0, Va);   // 1D: line 0 only
     }
   }
-  if (DST != DST_BUF && a.epi.dot_part) store_partial(a.epi.dot_part, acc);
+  if (DST != DST_BUF && a.epi.dot_part) store_partial(a.epi.dot_part, acc, tile);
+}
+
+template <typename T, int NT, int E, int D, int SRC, int DST, int MODE>
+__global__ __launch_bounds__(NT) void k_dim1(PassArgs<T> a) {
+  dim1_body<T, NT, E, D, SRC, DST, MODE>(a, blockIdx.x);
+}
+
+template <typename T, int NT, int E, int D, int SRC, int DST, int MODE>
+__global__ __launch_bounds__(NT) void kb_dim1(const PassArgs<T>* __restrict__ tab, BSeg seg) {
+  const int i = seg.inst[blockIdx.x];
+  dim1_body<T, NT, E, D, SRC, DST, MODE>(tab[i], blockIdx.x - seg.first[i]);
 }
 
 // beta <-> u (exact: a signed permutation), one thread per half-spectrum element.
@@ -592,229 +671,6 @@ __global__ __launch_bounds__(256) void k_precond_apply(const T* __restrict__ x, 
   y[n + i] = yz;
 }
 
-// CG control block, device resident.  status: 0 solved, 1 itmax, 2 zero curvature
-struct CgState {
-  double gamma;
-  double pq;
-  double alpha;
-  double beta;
-  double rnorm;
-  double eps;
-  double atol, rtol;
-  int64_t iters;
-  int64_t itmax;
-  int done;
-  int status;
-};
-
-// finalize a dot product from per-block partials (one block)
-__device__ __forceinline__ double sum_partials(const double* part, int nparts) {
-  double s = 0.0;
-  for (int i = threadIdx.x; i < nparts; i += blockDim.x) s += part[i];
-  return block_reduce(s, OpSum());
-}
-
-// The CG vectors have nb blocks of n: nb = 2 for the condensed system (the
-// block preconditioner [[P11, P12], [P12, P22]]), nb = 1 for the reduced
-// system (diagonal preconditioner P11; P12, P22 unused).
-// x = 0, r = b, z = P r, p = z; partial r'z
-template <typename T>
-__global__ __launch_bounds__(256) void k_cg_init(const T* __restrict__ b, T* x, T* r, T* z,
-                                                 T* p, const T* __restrict__ P11,
-                                                 const T* __restrict__ P12,
-                                                 const T* __restrict__ P22, int64_t n, int nb,
-                                                 double* part) {
-  double acc = 0.0;
-  const bool two = nb == 2;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const T rb = b[i];
-    if (!two) {
-      const T zb = P11[i] * rb;
-      x[i] = 0;
-      r[i] = rb;
-      z[i] = zb;
-      p[i] = zb;
-      acc += (double)rb * zb;
-      continue;
-    }
-    const T rz = b[n + i];
-    const T zb = P11[i] * rb + P12[i] * rz;
-    const T zz = P12[i] * rb + P22[i] * rz;
-    x[i] = 0;
-    x[n + i] = 0;
-    r[i] = rb;
-    r[n + i] = rz;
-    z[i] = zb;
-    z[n + i] = zz;
-    p[i] = zb;
-    p[n + i] = zz;
-    acc += (double)rb * zb + (double)rz * zz;
-  }
-  store_partial(part, acc);
-}
-
-__global__ void k_cg_start(CgState* st, const double* part, int nparts) {
-  const double g = sum_partials(part, nparts);
-  if (threadIdx.x == 0) {
-    st->gamma = g;
-    st->rnorm = sqrt(g > 0 ? g : 0.0);
-    st->eps = st->atol + st->rtol * st->rnorm;
-    st->iters = 0;
-    st->status = 1;
-    st->done = 0;
-    if (st->rnorm <= st->eps) {
-      st->done = 1;
-      st->status = 0;
-    }
-  }
-}
-
-// after q = K p: alpha = gamma / p'q (zero curvature stops)
-__global__ void k_cg_alpha(CgState* st, const double* part, int nparts) {
-  if (st->done) return;
-  const double pq = sum_partials(part, nparts);
-  if (threadIdx.x == 0) {
-    st->pq = pq;
-    if (!(pq > 0.0)) {
-      st->done = 1;
-      st->status = 2;
-    } else {
-      st->alpha = st->gamma / pq;
-    }
-  }
-}
-
-// x += alpha p; r -= alpha q; z = P r; partial r'z
-template <typename T>
-__global__ __launch_bounds__(256) void k_cg_update(T* x, T* r, T* z, const T* __restrict__ p,
-                                                   const T* __restrict__ q,
-                                                   const T* __restrict__ P11,
-                                                   const T* __restrict__ P12,
-                                                   const T* __restrict__ P22, int64_t n,
-                                                   int nb, const CgState* st, double* part) {
-  if (st->done) return;
-  const T alpha = (T)st->alpha;
-  double acc = 0.0;
-  const bool two = nb == 2;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    if (!two) {
-      x[i] += alpha * p[i];
-      const T rb = r[i] - alpha * q[i];
-      const T zb = P11[i] * rb;
-      r[i] = rb;
-      z[i] = zb;
-      acc += (double)rb * zb;
-      continue;
-    }
-    x[i] += alpha * p[i];
-    x[n + i] += alpha * p[n + i];
-    const T rb = r[i] - alpha * q[i];
-    const T rz = r[n + i] - alpha * q[n + i];
-    r[i] = rb;
-    r[n + i] = rz;
-    const T zb = P11[i] * rb + P12[i] * rz;
-    const T zz = P12[i] * rb + P22[i] * rz;
-    z[i] = zb;
-    z[n + i] = zz;
-    acc += (double)rb * zb + (double)rz * zz;
-  }
-  store_partial(part, acc);
-}
-
-__global__ void k_cg_gamma(CgState* st, const double* part, int nparts) {
-  if (st->done) return;
-  const double g = sum_partials(part, nparts);
-  if (threadIdx.x == 0) {
-    st->iters += 1;
-    st->rnorm = sqrt(g > 0 ? g : 0.0);
-    st->beta = g / st->gamma;
-    st->gamma = g;
-    if (st->rnorm <= st->eps) {
-      st->done = 1;
-      st->status = 0;
-    } else if (st->iters >= st->itmax) {
-      st->done = 1;
-      st->status = 1;
-    }
-  }
-}
-
-// reliable update: r = b - q with q = K x just recomputed, z = P r, partial r'z
-template <typename T>
-__global__ __launch_bounds__(256) void k_cg_residual(const T* __restrict__ b,
-                                                     const T* __restrict__ q, T* r, T* z,
-                                                     const T* __restrict__ P11,
-                                                     const T* __restrict__ P12,
-                                                     const T* __restrict__ P22, int64_t n,
-                                                     int nb, const CgState* st, double* part) {
-  if (st->done) return;
-  double acc = 0.0;
-  const bool two = nb == 2;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    if (!two) {
-      const T rb = b[i] - q[i];
-      const T zb = P11[i] * rb;
-      r[i] = rb;
-      z[i] = zb;
-      acc += (double)rb * zb;
-      continue;
-    }
-    const T rb = b[i] - q[i], rz = b[n + i] - q[n + i];
-    const T zb = P11[i] * rb + P12[i] * rz;
-    const T zz = P12[i] * rb + P22[i] * rz;
-    r[i] = rb;
-    r[n + i] = rz;
-    z[i] = zb;
-    z[n + i] = zz;
-    acc += (double)rb * zb + (double)rz * zz;
-  }
-  store_partial(part, acc);
-}
-
-// after a reliable update: gamma from the true residual, restart p = z
-__global__ void k_cg_regamma(CgState* st, const double* part, int nparts) {
-  if (st->done) return;
-  const double g = sum_partials(part, nparts);
-  if (threadIdx.x == 0) {
-    st->rnorm = sqrt(g > 0 ? g : 0.0);
-    st->gamma = g;
-    st->beta = 0.0;
-    if (st->rnorm <= st->eps) {
-      st->done = 1;
-      st->status = 0;
-    }
-  }
-}
-
-// p = z + beta p
-template <typename T>
-__global__ __launch_bounds__(256) void k_cg_pupdate(T* p, const T* __restrict__ z, int64_t n2,
-                                                    const CgState* st) {
-  if (st->done) return;
-  const T beta = (T)st->beta;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n2;
-       i += (int64_t)gridDim.x * blockDim.x)
-    p[i] = z[i] + beta * p[i];
-}
-
-
-// ===========================================================================
-// Interior-point driver kernels (FP64 control).  State: beta, c, duals z1, z2
-// of the slacks t1 = c + beta >= 0, t2 = c - beta >= 0 of FFTNLPModel's
-// constraints -beta - c <= 0, beta - c <= 0 (src/fft_model.jl:92-101).
-// ===========================================================================
-struct RedSpec {
-  int nq;
-  int op[8];  // 0 sum, 1 max, 2 min
-};
-
-__device__ __forceinline__ double red_ident(int op) {
-  return op == 0 ? 0.0 : (op == 1 ? -1e300 : 1e300);
-}
-
 // per-block partials of up to 8 quantities: part[q * gridDim.x + block]
 template <int NQ>
 __device__ __forceinline__ void store_partials(double* part, const double (&acc)[NQ],
@@ -827,330 +683,6 @@ __device__ __forceinline__ void store_partials(double* part, const double (&acc)
     else r = block_reduce(acc[q], OpMin());
     if (threadIdx.x == 0) part[q * gridDim.x + blockIdx.x] = r;
   }
-}
-
-// ipm scalar slots
-enum {
-  S_RB = 0, S_RC, S_TZ, S_BQB, S_BB, S_SUMC,  // residual pass
-  S_MU, S_AP_AFF, S_AD_AFF, S_SIGMA, S_AP, S_AD, S_BNORM, S_NSLOTS = 16
-};
-
-// mode 0: out[q] = reductions (+ mu from S_TZ); 1: affine step lengths;
-// 2: sigma = min(1, (mu_aff/mu)^3); 3: corrector step lengths with tau
-__global__ void k_ipm_finalize(const double* part, int nparts, RedSpec spec, double* scal,
-                               int mode, int first, double inv2n, double tau) {
-  for (int q = 0; q < spec.nq; ++q) {
-    const int op = spec.op[q];
-    double acc = red_ident(op);
-    for (int i = threadIdx.x; i < nparts; i += blockDim.x) {
-      const double v = part[q * nparts + i];
-      acc = op == 0 ? acc + v : (op == 1 ? fmax(acc, v) : fmin(acc, v));
-    }
-    double r;
-    if (op == 0) r = block_reduce(acc, OpSum());
-    else if (op == 1) r = block_reduce(acc, OpMax());
-    else r = block_reduce(acc, OpMin());
-    if (threadIdx.x == 0) {
-      if (mode == 0) scal[first + q] = r;
-      else if (mode == 1) scal[S_AP_AFF + q] = fmin(1.0, r);
-      else if (mode == 2) {
-        const double mu_aff = r * inv2n;
-        const double ratio = mu_aff / scal[S_MU];
-        scal[S_SIGMA] = fmin(1.0, ratio * ratio * ratio);
-      } else {
-        scal[S_AP + q] = fmin(1.0, tau * r);
-      }
-    }
-    __syncthreads();
-  }
-  if (mode == 0 && threadIdx.x == 0 && first == S_RB) scal[S_MU] = scal[S_TZ] * inv2n;
-}
-
-// slots with valid == null or bit set hold a beta entry; the others (padding
-// of the spectrum-ordered layout) stay inert: beta = 0, c = 1, z = 0
-#define FOR_SLOTS(i, n)                                                          \
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (n);      \
-       i += (int64_t)gridDim.x * blockDim.x)
-#define VALID(v, i) (!(v) || slot_valid((v), (i)))
-
-__global__ __launch_bounds__(256) void k_ipm_init(double* beta, double* c, double* z1, double* z2,
-                                                  int64_t n, const double* scal, double lam,
-                                                  const uint32_t* valid) {
-  const double cval = 1.0 + scal[S_BNORM];
-  const double zval = lam > 0 ? 0.5 * lam : 1.0;
-  FOR_SLOTS(i, n) {
-    const bool ok = VALID(valid, i);
-    beta[i] = 0.0;
-    c[i] = ok ? cval : 1.0;
-    z1[i] = ok ? zval : 0.0;
-    z2[i] = ok ? zval : 0.0;
-  }
-}
-
-__global__ __launch_bounds__(256) void k_absmax(const double* __restrict__ x, int64_t n,
-                                                double* part) {
-  double a[1] = {0.0};
-  FOR_SLOTS(i, n) a[0] = fmax(a[0], fabs(x[i]));
-  const int ops[1] = {1};
-  store_partials<1>(part, a, ops);
-}
-
-// residuals and objective pieces: max|r_b|, max|r_c|, sum t'z, beta'Qbeta, beta'b, sum c
-__global__ __launch_bounds__(256) void k_ipm_resid(const double* __restrict__ beta,
-                                                   const double* __restrict__ c,
-                                                   const double* __restrict__ z1,
-                                                   const double* __restrict__ z2,
-                                                   const double* __restrict__ Qb,
-                                                   const double* __restrict__ b, double lam,
-                                                   int64_t n, double* part,
-                                                   const uint32_t* valid) {
-  double a[6] = {0, 0, 0, 0, 0, 0};
-  FOR_SLOTS(i, n) {
-    if (!VALID(valid, i)) continue;
-    const double be = beta[i], ce = c[i], y1 = z1[i], y2 = z2[i], q = Qb[i], bb = b[i];
-    const double t1 = ce + be, t2 = ce - be;
-    const double rb = q - bb - y1 + y2;
-    const double rc = lam - y1 - y2;
-    a[0] = fmax(a[0], fabs(rb));
-    a[1] = fmax(a[1], fabs(rc));
-    a[2] += t1 * y1 + t2 * y2;
-    a[3] += be * q;
-    a[4] += be * bb;
-    a[5] += ce;
-  }
-  const int ops[6] = {1, 1, 0, 0, 0, 0};
-  store_partials<6>(part, a, ops);
-}
-
-// slack diagonals, preconditioner (in the CG precision TC) and the predictor
-// right-hand side [-r_b + g1 - g2; -r_c + g1 + g2] with g_i = -z_i
-template <typename TC>
-__global__ __launch_bounds__(256) void k_ipm_prep(
-    const double* __restrict__ beta, const double* __restrict__ c, const double* __restrict__ z1,
-    const double* __restrict__ z2, const double* __restrict__ Qb, const double* __restrict__ b,
-    double lam, int64_t n, double* s1o, double* s2o, TC* L1, TC* L2, TC* P11, TC* P12, TC* P22,
-    TC* rhs, const uint32_t* valid) {
-  FOR_SLOTS(i, n) {
-    double s1 = 0, s2 = 0, rb = 0, rc = 0, y1 = 0, y2 = 0;
-    double l1 = 0, l2 = 0, p11 = 1, p12 = 0, p22 = 1;
-    const bool ok = VALID(valid, i);
-    if (ok) {
-      const double be = beta[i], ce = c[i];
-      y1 = z1[i];
-      y2 = z2[i];
-      const double t1 = ce + be, t2 = ce - be;
-      s1 = y1 / t1;
-      s2 = y2 / t2;
-      precond_entries<double>(s1, s2, l1, l2, p11, p12, p22);
-      rb = Qb[i] - b[i] - y1 + y2;
-      rc = lam - y1 - y2;
-    }
-    s1o[i] = s1;
-    s2o[i] = s2;
-    L1[i] = (TC)l1;
-    L2[i] = (TC)l2;
-    P11[i] = (TC)p11;
-    P12[i] = (TC)p12;
-    P22[i] = (TC)p22;
-    rhs[i] = (TC)(ok ? -rb - y1 + y2 : 0.0);
-    rhs[n + i] = (TC)(ok ? -rc - y1 - y2 : 0.0);
-  }
-}
-
-// The same for the reduced system: L1 = S1 + S2, L2 = S1 - S2 (FP64),
-// D = 4 S1 S2 / (S1 + S2) (FP64, and in TC with the preconditioner 1/(1 + D)),
-// rhsd = [h_b - (L2/L1) h_c; h_c] (FP64) and rhs = h_b - (L2/L1) h_c in TC.
-template <typename TC>
-__global__ __launch_bounds__(256) void k_ipm_prep_red(
-    const double* __restrict__ beta, const double* __restrict__ c, const double* __restrict__ z1,
-    const double* __restrict__ z2, const double* __restrict__ Qb, const double* __restrict__ b,
-    double lam, int64_t n, double* s1o, double* s2o, TC* D, TC* P, TC* rhs, double* L1d,
-    double* L2d, double* Dd, double* rhsd, const uint32_t* valid) {
-  FOR_SLOTS(i, n) {
-    double s1 = 0, s2 = 0, l1 = 0, l2 = 0, dg = 0, hb = 0, hc = 0;
-    if (VALID(valid, i)) {
-      const double be = beta[i], ce = c[i], y1 = z1[i], y2 = z2[i];
-      const double t1 = ce + be, t2 = ce - be;
-      s1 = y1 / t1;
-      s2 = y2 / t2;
-      l1 = s1 + s2;
-      l2 = s1 - s2;
-      dg = 4.0 * s1 * s2 / l1;
-      const double rb = Qb[i] - b[i] - y1 + y2;
-      const double rc = lam - y1 - y2;
-      hb = -rb - y1 + y2;
-      hc = -rc - y1 - y2;
-      hb -= (l2 / l1) * hc;
-    }
-    s1o[i] = s1;
-    s2o[i] = s2;
-    L1d[i] = l1;
-    L2d[i] = l2;
-    Dd[i] = dg;
-    D[i] = (TC)dg;
-    P[i] = (TC)(1.0 / (1.0 + dg));
-    rhsd[i] = hb;
-    rhsd[n + i] = hc;
-    rhs[i] = (TC)hb;
-  }
-}
-
-// dc = (h_c - L2 db) / L1 into the second block of d (FP64)
-__global__ __launch_bounds__(256) void k_red_back(double* d, const double* __restrict__ rhsd,
-                                                  const double* __restrict__ L1d,
-                                                  const double* __restrict__ L2d, int64_t n,
-                                                  const uint32_t* valid) {
-  FOR_SLOTS(i, n) d[n + i] = VALID(valid, i) ? (rhsd[n + i] - L2d[i] * d[i]) / L1d[i] : 0.0;
-}
-
-// affine direction: dt, dz and the largest steps to the boundary
-template <typename TC>
-__global__ __launch_bounds__(256) void k_ipm_dir_aff(
-    const double* __restrict__ beta, const double* __restrict__ c, const double* __restrict__ z1,
-    const double* __restrict__ z2, const double* __restrict__ s1, const double* __restrict__ s2,
-    const TC* __restrict__ d, int64_t n, double* dt1o, double* dt2o, double* dz1o, double* dz2o,
-    double* part, const uint32_t* valid) {
-  double a[2] = {1e300, 1e300};
-  FOR_SLOTS(i, n) {
-    if (!VALID(valid, i)) {
-      dt1o[i] = dt2o[i] = dz1o[i] = dz2o[i] = 0.0;
-      continue;
-    }
-    const double be = beta[i], ce = c[i], y1 = z1[i], y2 = z2[i];
-    const double t1 = ce + be, t2 = ce - be;
-    const double db = (double)d[i], dc = (double)d[n + i];
-    const double dt1 = dc + db, dt2 = dc - db;
-    const double dz1 = -y1 - s1[i] * dt1;
-    const double dz2 = -y2 - s2[i] * dt2;
-    dt1o[i] = dt1;
-    dt2o[i] = dt2;
-    dz1o[i] = dz1;
-    dz2o[i] = dz2;
-    if (dt1 < 0) a[0] = fmin(a[0], -t1 / dt1);
-    if (dt2 < 0) a[0] = fmin(a[0], -t2 / dt2);
-    if (dz1 < 0) a[1] = fmin(a[1], -y1 / dz1);
-    if (dz2 < 0) a[1] = fmin(a[1], -y2 / dz2);
-  }
-  const int ops[2] = {2, 2};
-  store_partials<2>(part, a, ops);
-}
-
-// complementarity after the affine step
-__global__ __launch_bounds__(256) void k_ipm_muaff(
-    const double* __restrict__ beta, const double* __restrict__ c, const double* __restrict__ z1,
-    const double* __restrict__ z2, const double* __restrict__ dt1, const double* __restrict__ dt2,
-    const double* __restrict__ dz1, const double* __restrict__ dz2, const double* scal, int64_t n,
-    double* part, const uint32_t* valid) {
-  const double ap = scal[S_AP_AFF], ad = scal[S_AD_AFF];
-  double a[1] = {0.0};
-  FOR_SLOTS(i, n) {
-    if (!VALID(valid, i)) continue;
-    const double be = beta[i], ce = c[i];
-    const double t1 = ce + be, t2 = ce - be;
-    a[0] += (t1 + ap * dt1[i]) * (z1[i] + ad * dz1[i]) + (t2 + ap * dt2[i]) * (z2[i] + ad * dz2[i]);
-  }
-  const int ops[1] = {0};
-  store_partials<1>(part, a, ops);
-}
-
-// corrector right-hand side: g_i = (sigma mu - t_i z_i - dt_i^aff dz_i^aff) / t_i
-template <typename TC>
-__global__ __launch_bounds__(256) void k_ipm_rhs_corr(
-    const double* __restrict__ beta, const double* __restrict__ c, const double* __restrict__ z1,
-    const double* __restrict__ z2, const double* __restrict__ Qb, const double* __restrict__ b,
-    const double* __restrict__ dt1, const double* __restrict__ dt2,
-    const double* __restrict__ dz1, const double* __restrict__ dz2, const double* scal,
-    double lam, int64_t n, TC* rhs, double* rhsd, const double* __restrict__ L1d,
-    const double* __restrict__ L2d, const uint32_t* valid) {
-  const double smu = scal[S_SIGMA] * scal[S_MU];
-  FOR_SLOTS(i, n) {
-    double hb = 0, hc = 0;
-    if (VALID(valid, i)) {
-      const double be = beta[i], ce = c[i], y1 = z1[i], y2 = z2[i];
-      const double t1 = ce + be, t2 = ce - be;
-      const double g1 = (smu - t1 * y1 - dt1[i] * dz1[i]) / t1;
-      const double g2 = (smu - t2 * y2 - dt2[i] * dz2[i]) / t2;
-      const double rb = Qb[i] - b[i] - y1 + y2;
-      const double rc = lam - y1 - y2;
-      hb = -rb + g1 - g2;
-      hc = -rc + g1 + g2;
-    }
-    if (rhsd) {   // reduced system (see k_ipm_prep_red)
-      if (VALID(valid, i)) hb -= (L2d[i] / L1d[i]) * hc;
-      rhsd[i] = hb;
-      rhsd[n + i] = hc;
-      rhs[i] = (TC)hb;
-    } else {
-      rhs[i] = (TC)hb;
-      rhs[n + i] = (TC)hc;
-    }
-  }
-}
-
-// corrector direction (overwrites the affine dt, dz) and boundary steps
-template <typename TC>
-__global__ __launch_bounds__(256) void k_ipm_dir(
-    const double* __restrict__ beta, const double* __restrict__ c, const double* __restrict__ z1,
-    const double* __restrict__ z2, const double* __restrict__ s1, const double* __restrict__ s2,
-    const TC* __restrict__ d, const double* scal, int64_t n, double* dt1o, double* dt2o,
-    double* dz1o, double* dz2o, double* part, const uint32_t* valid) {
-  const double smu = scal[S_SIGMA] * scal[S_MU];
-  double a[2] = {1e300, 1e300};
-  FOR_SLOTS(i, n) {
-    if (!VALID(valid, i)) {
-      dt1o[i] = dt2o[i] = dz1o[i] = dz2o[i] = 0.0;
-      continue;
-    }
-    const double be = beta[i], ce = c[i], y1 = z1[i], y2 = z2[i];
-    const double t1 = ce + be, t2 = ce - be;
-    const double g1 = (smu - t1 * y1 - dt1o[i] * dz1o[i]) / t1;
-    const double g2 = (smu - t2 * y2 - dt2o[i] * dz2o[i]) / t2;
-    const double db = (double)d[i], dc = (double)d[n + i];
-    const double dt1 = dc + db, dt2 = dc - db;
-    const double dz1 = g1 - s1[i] * dt1;
-    const double dz2 = g2 - s2[i] * dt2;
-    dt1o[i] = dt1;
-    dt2o[i] = dt2;
-    dz1o[i] = dz1;
-    dz2o[i] = dz2;
-    if (dt1 < 0) a[0] = fmin(a[0], -t1 / dt1);
-    if (dt2 < 0) a[0] = fmin(a[0], -t2 / dt2);
-    if (dz1 < 0) a[1] = fmin(a[1], -y1 / dz1);
-    if (dz2 < 0) a[1] = fmin(a[1], -y2 / dz2);
-  }
-  const int ops[2] = {2, 2};
-  store_partials<2>(part, a, ops);
-}
-
-template <typename TC>
-__global__ __launch_bounds__(256) void k_ipm_update(double* beta, double* c, double* z1,
-                                                    double* z2, const TC* __restrict__ d,
-                                                    const double* __restrict__ dz1,
-                                                    const double* __restrict__ dz2,
-                                                    const double* scal, int64_t n,
-                                                    const uint32_t* valid) {
-  const double ap = scal[S_AP], ad = scal[S_AD];
-  FOR_SLOTS(i, n) {
-    if (!VALID(valid, i)) continue;
-    beta[i] += ap * (double)d[i];
-    c[i] += ap * (double)d[n + i];
-    z1[i] += ad * dz1[i];
-    z2[i] += ad * dz2[i];
-  }
-}
-
-// mixed precision helpers: x_lo = (TC) x_hi;  x_hi += (double) x_lo
-template <typename TC>
-__global__ __launch_bounds__(256) void k_narrow(const double* __restrict__ x, TC* y, int64_t n) {
-  FOR_SLOTS(i, n) y[i] = (TC)x[i];
-}
-template <typename TC>
-__global__ __launch_bounds__(256) void k_widen_add(double* y, const TC* __restrict__ x, int64_t n) {
-  FOR_SLOTS(i, n) y[i] += (double)x[i];
-}
-template <typename TC>
-__global__ __launch_bounds__(256) void k_widen(double* y, const TC* __restrict__ x, int64_t n) {
-  FOR_SLOTS(i, n) y[i] = (double)x[i];
 }
 
 // ===========================================================================

@@ -170,6 +170,59 @@ int csipm_solve(csipm_op_t h, const double* z0, double lambda, double tol,
                 int64_t max_iter, double cg_rtol, int64_t cg_itmax,
                 double* beta_out, double* info);
 
+/* ------------------------------------------------------------------------
+ * Batch engine: many solves of uneven sizes and dimensionalities in the same
+ * kernel launches (one launch per pass / vector step for all instances),
+ * with per-instance convergence masks, and continuous batching: instances
+ * are admitted and retired between interior-point iterations while the
+ * others keep running.  No counterpart in the reference, which solves one
+ * problem per MadNLP call (src/CompressedSensingIPM.jl:31-36); every
+ * instance follows csipm_solve's algorithm and stopping test.
+ *
+ * precision: CSIPM_F64 (FP64 condensed-K CG, as csipm_solve on an F64
+ * handle) or CSIPM_MIXED (reduced system, FP32 operator and CG, FP64
+ * control and refinement).  Admitted handles must have that precision.
+ * slots: pool size in spectrum-order slots (csipm_batch_slots per instance);
+ * device memory is about csipm_batch_bytes(precision, slots).
+ * All calls run on `stream` and return when their work is done.
+ * ------------------------------------------------------------------------ */
+typedef struct csipm_batch* csipm_batch_t;
+
+int csipm_batch_create(csipm_batch_t* out, int precision, int64_t slots,
+                       int max_instances, void* stream);
+int csipm_batch_destroy(csipm_batch_t b);
+/* pool slots instance h occupies, and pool bytes of a batch of `slots` */
+int csipm_batch_slots(csipm_op_t h, int64_t* slots);
+int csipm_batch_bytes(int precision, int64_t slots, int64_t* bytes);
+/* Admit a problem: z0 (device, N doubles) and beta_out (device, N doubles)
+ * stay owned by the caller and must live until the instance is harvested.
+ * CSIPM_ERR_ALLOC when the pool has no free range or all ids are taken.  */
+int csipm_batch_admit(csipm_batch_t b, csipm_op_t h, const double* z0,
+                      double lambda, double* beta_out, int64_t cg_itmax, int* id);
+/* One interior-point iteration of every admitted, unfinished instance
+ * (newly admitted ones are initialised first).  *running: instances still
+ * iterating.  Finished ones are listed by csipm_batch_poll.              */
+int csipm_batch_step(csipm_batch_t b, double tol, int64_t max_iter,
+                     double cg_rtol, int* running);
+int csipm_batch_poll(csipm_batch_t b, int* ids, int cap, int* n);
+/* Write a finished instance's beta into its beta_out, fill info[8] as
+ * csipm_solve does (seconds = 0) and free its slots and id.              */
+int csipm_batch_harvest(csipm_batch_t b, int id, double* info);
+/* Convenience: admit n problems, iterate until all finish, harvest all.
+ * info: n * 8 doubles.                                                   */
+int csipm_batch_solve(csipm_batch_t b, int n, const csipm_op_t* ops,
+                      const double* const* z0, const double* lambda, double tol,
+                      int64_t max_iter, double cg_rtol, int64_t cg_itmax,
+                      double* const* beta_out, double* info);
+/* Timing of the batch's CG operator apply (one launch per pass and
+ * dimensionality over all admitted instances, vectors zeroed): HIP events
+ * around every launch, averaged over nrep applies; pass_bytes receives the
+ * algorithmic HBM bytes of each launch.  cap: length of the output arrays.
+ * csipm_batch_launch_name names launch k ("<dim>d:<pass>").              */
+int csipm_batch_apply_timed(csipm_batch_t b, int64_t nrep, double* pass_ms,
+                            double* pass_bytes, int cap, int* nlaunch);
+const char* csipm_batch_launch_name(csipm_batch_t b, int k);
+
 #ifdef __cplusplus
 }
 #endif

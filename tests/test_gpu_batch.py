@@ -1,0 +1,106 @@
+"""GPU tests of the batch engine (csipm_batch_*).
+
+* A batch of uneven 1D / 2D / 3D problems gives, instance by instance, the
+  same beta, objective and iteration counts as the same problem solved alone
+  (csipm_solve, a batch of one): the kernels and reduction orders are the
+  same, so the check is bitwise.
+* Continuous batching: problems admitted while others are mid-solve, and
+  retired as they finish, still give the alone results.
+* The mixed-precision batch reaches the FP64 oracle objective within 1e-6.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import ipm
+from oracle import operator as op
+
+pytestmark = pytest.mark.gpu
+
+
+def _problems():
+    import csipm.workloads as wl
+    return [wl.reference_example((128,), False),
+            wl.batch_instance((1500,), seed=21),
+            wl.batch_instance((63, 63), seed=22),
+            wl.batch_instance((26, 21, 15), seed=23),
+            wl.batch_instance((61, 53), seed=24),
+            wl.batch_instance((517,), seed=25),
+            wl.reference_example((10, 8, 12), False)]
+
+
+def _op(p, mixed):
+    from csipm.fft_operator import FFTOperator
+    return FFTOperator(p.N, len(p.dims), p.dims, p.missing, mixed=mixed)
+
+
+def _alone(p, mixed, tol):
+    from csipm.fft_operator import FFTParameters
+    from csipm.model import FFTNLPModel, ipm_solve
+    params = FFTParameters(len(p.dims), p.dims, p.z0, p.lam, p.missing)
+    return ipm_solve(FFTNLPModel(params, mixed=mixed), tol=tol)
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_batch_equals_alone(cuda, mixed):
+    from csipm.batch import solve_batch
+    print("batch vs alone", "mixed" if mixed else "f64", flush=True)
+    tol = 1e-7 if mixed else 1e-8
+    probs = _problems()
+    ops = [_op(p, mixed) for p in probs]
+    res = solve_batch(ops, [p.z0 for p in probs], [p.lam for p in probs], mixed=mixed, tol=tol)
+    for p, r in zip(probs, res):
+        a = _alone(p, mixed, tol)
+        assert r.status == "solved" and a.status == "solved", p.name
+        assert r.iterations == a.iterations and r.cg_iterations == a.cg_iterations, p.name
+        assert r.objective == a.objective, p.name
+        assert torch.equal(r.beta, a.solution), p.name
+
+
+def test_continuous_batching(cuda):
+    from csipm.batch import BatchSolver, slots_of
+    print("continuous batching", flush=True)
+    probs = _problems()
+    ops = [_op(p, True) for p in probs]
+    # a pool too small for all at once: later problems wait for slots
+    cap = sum(sorted(slots_of(o) for o in ops)[-3:])
+    bs = BatchSolver(cap, max_instances=4, mixed=True)
+    queue = list(range(len(probs)))
+    done = {}
+    steps = 0
+    while queue or bs.live:
+        while queue:
+            k = queue[0]
+            if bs.admit(ops[k], probs[k].z0, probs[k].lam, tag=k) is None:
+                break
+            queue.pop(0)
+        assert bs.live > 0
+        bs.step(tol=1e-7)
+        steps += 1
+        for i in bs.finished():
+            r = bs.harvest(i)
+            done[r.tag] = r
+        assert steps < 2000
+    bs.close()
+    assert sorted(done) == list(range(len(probs)))
+    for k, p in enumerate(probs):
+        a = _alone(p, True, 1e-7)
+        r = done[k]
+        assert r.iterations == a.iterations and r.objective == a.objective, p.name
+        assert torch.equal(r.beta, a.solution), p.name
+
+
+def test_mixed_batch_vs_oracle(cuda):
+    from csipm.batch import solve_batch
+    print("mixed batch vs oracle", flush=True)
+    probs = _problems()
+    ops = [_op(p, True) for p in probs]
+    res = solve_batch(ops, [p.z0 for p in probs], [p.lam for p in probs], mixed=True, tol=1e-7)
+    for p, r in zip(probs, res):
+        b = op.m_perpt_z(p.z0, p.dims)
+        mt = lambda v: op.mtm(v, p.dims, p.missing)    # noqa: E731
+        ref = ipm.solve(b, p.lam, mt, ipm.IPMOptions(tol=1e-10, cg_rtol=1e-13))
+        beta = r.beta.cpu().numpy()
+        f = ipm.objective(beta, np.abs(beta), mt(beta), b, p.lam)
+        f_ref = ipm.objective(ref.beta, np.abs(ref.beta), mt(ref.beta), b, p.lam)
+        assert abs(f - f_ref) <= 1e-6 * abs(f_ref), p.name

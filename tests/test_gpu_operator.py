@@ -64,7 +64,11 @@ def test_operator_golden(cuda, fname):
     assert _rel(M_perpt_z(o, z).cpu().numpy(), d["m_perpt_z"]) <= TOL64
 
 
-@pytest.mark.parametrize("dims", [(1024,), (255, 255), (64, 48, 30), (51, 45, 35)])
+# sizes with a prime factor above 17 run through Bluestein line transforms
+BLUESTEIN = [(127,), (4001,), (19, 23), (127, 60), (23, 19, 29), (12, 10, 37)]
+
+
+@pytest.mark.parametrize("dims", [(1024,), (255, 255), (64, 48, 30), (51, 45, 35)] + BLUESTEIN)
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
 def test_operator_masked_vs_oracle(cuda, dims, dtype):
     from csipm.fft_operator import M_perp_beta, M_perpt_M_perp_vec, M_perpt_z
@@ -116,7 +120,7 @@ def test_kkt_and_precond(cuda, dims, dtype):
     assert _rel(y.double().cpu().numpy(), want) <= 10 * tol
 
 
-@pytest.mark.parametrize("dims", [(1024,), (255, 255), (33, 30, 26)])
+@pytest.mark.parametrize("dims", [(1024,), (255, 255), (33, 30, 26), (127, 60)])
 def test_cg_solve(cuda, dims):
     import ctypes
     from csipm import _lib

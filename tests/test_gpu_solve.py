@@ -100,7 +100,8 @@ def _support(beta):
 
 
 @pytest.mark.parametrize("mixed", [False, True])
-@pytest.mark.parametrize("case", ["ex1d", "ex2d", "ex3d", "cfg0", "b1d", "b2d", "b3d"])
+@pytest.mark.parametrize("case", ["ex1d", "ex2d", "ex3d", "cfg0", "b1d", "b2d", "b3d", "p1d",
+                                  "p2d"])
 def test_solve_vs_oracle(cuda, case, mixed):
     import csipm.workloads as wl
     from csipm.model import ipm_solve
@@ -111,7 +112,9 @@ def test_solve_vs_oracle(cuda, case, mixed):
          "cfg0": lambda: wl.config0(),
          "b1d": lambda: wl.batch_instance((1500,), seed=11),
          "b2d": lambda: wl.batch_instance((63, 63), seed=12),
-         "b3d": lambda: wl.batch_instance((26, 21, 15), seed=13)}[case]()
+         "b3d": lambda: wl.batch_instance((26, 21, 15), seed=13),
+         "p1d": lambda: wl.batch_instance((2003,), seed=14),
+         "p2d": lambda: wl.batch_instance((61, 53), seed=15)}[case]()
     b = op.m_perpt_z(p.z0, p.dims)
     mt = lambda v: op.mtm(v, p.dims, p.missing)    # noqa: E731
     ref = ipm.solve(b, p.lam, mt, ipm.IPMOptions(tol=1e-10, cg_rtol=1e-13))
