@@ -239,13 +239,14 @@ static int upload_twiddles(int L, void** out) {
 
 // line FFTs along dim 2/3: 2^lg lines of length L per workgroup, as many as
 // the registers (lines*L <= NT*E) and 64 KB of LDS allow, fewer when that
-// would leave the chip with under ~1024 workgroups
-static LineCfg line_cfg(int H1, int L, int64_t others, size_t cs) {
+// would leave the launch with under min_grid workgroups (1024 for one
+// instance; a batch shares the target among its instances)
+static LineCfg line_cfg(int H1, int L, int64_t others, size_t cs, int min_grid = 1024) {
   LineCfg c{};
   int lg = 0;
   while ((2 << lg) * L <= kNT * kE && (size_t)(2 << lg) * L * cs <= kLdsSmall && (1 << lg) < H1)
     ++lg;
-  while (lg > 0 && (int64_t)((H1 + (1 << lg) - 1) >> lg) * others < 1024) --lg;
+  while (lg > 0 && (int64_t)((H1 + (1 << lg) - 1) >> lg) * others < min_grid) --lg;
   c.lg = lg;
   c.ntile1 = (H1 + (1 << lg) - 1) >> lg;
   c.grid = (int)(c.ntile1 * others);
@@ -253,14 +254,14 @@ static LineCfg line_cfg(int H1, int L, int64_t others, size_t cs) {
   return c;
 }
 
-static Dim1Cfg dim1_cfg(int N1, int nlines, size_t cs) {
+static Dim1Cfg dim1_cfg(int N1, int nlines, size_t cs, int min_grid = 1024) {
   Dim1Cfg c{};
   c.npairs = (nlines + 1) / 2;
   c.nt = (N1 <= kNT * kE) ? kNT : kNTBig;
   int lg = 0;
   auto lds_of = [&](int l) { return (size_t)N1 * ((1 << l) + (l > 0 ? 1 : 0)) * cs; };
   while ((2 << lg) * N1 <= c.nt * kE && lds_of(lg + 1) <= kLdsSmall && (1 << lg) < c.npairs) ++lg;
-  while (lg > 0 && ((c.npairs + (1 << lg) - 1) >> lg) < 1024) --lg;
+  while (lg > 0 && ((c.npairs + (1 << lg) - 1) >> lg) < min_grid) --lg;
   c.lg = lg;
   c.ps = (1 << lg) + (lg > 0 ? 1 : 0);
   c.grid = (c.npairs + (1 << lg) - 1) >> lg;
@@ -310,8 +311,8 @@ PassArgs<T> base_args(csipm_op* h, int axis) {
 
 template <typename T>
 PassArgs<T> lines_args(csipm_op* h, int axis, const void* src, const Epi<T>& epi,
-                       const int* done) {
-  const LineCfg& c = h->ps<T>().lines[axis];
+                       const int* done, const LineCfg* cfg = nullptr) {
+  const LineCfg& c = cfg ? *cfg : h->ps<T>().lines[axis];
   PassArgs<T> a = base_args<T>(h, axis);
   a.axis = axis;
   a.lg = c.lg;
@@ -326,8 +327,9 @@ PassArgs<T> lines_args(csipm_op* h, int axis, const void* src, const Epi<T>& epi
 
 template <typename T>
 PassArgs<T> dim1_args(csipm_op* h, const void* src, const T* real_in, T* real_out,
-                      const Epi<T>& epi, T scale, const int* done) {
-  const Dim1Cfg& c = h->ps<T>().d1;
+                      const Epi<T>& epi, T scale, const int* done,
+                      const Dim1Cfg* cfg = nullptr) {
+  const Dim1Cfg& c = cfg ? *cfg : h->ps<T>().d1;
   PassArgs<T> a = base_args<T>(h, 0);
   a.lg = c.lg;
   a.ps = c.ps;

@@ -314,10 +314,28 @@ int vblocks(int64_t n) {
   return (int)std::max<int64_t>(1, std::min<int64_t>(b, 2048));
 }
 
+// launch shapes of instance h inside a chain over `members` instances: the
+// workgroup-count target of a single launch is shared among them
+struct BatchCfg {
+  LineCfg lines[3];
+  Dim1Cfg d1;
+};
+
+template <typename T>
+BatchCfg batch_cfg(csipm_op* h, int members) {
+  const int target = std::max(1, 1024 / std::max(1, members));
+  const size_t cs = sizeof(cpx<T>);
+  BatchCfg b{};
+  b.d1 = dim1_cfg(h->plan[0].L, (int)(h->N / h->n[0]), cs, target);
+  if (h->dim >= 2) b.lines[1] = line_cfg(h->H1, h->plan[1].L, h->n[2], cs, target);
+  if (h->dim == 3) b.lines[2] = line_cfg(h->H1, h->plan[2].L, h->n[1], cs, target);
+  return b;
+}
+
 // per-instance epilogue / source of a chain call
 template <typename T>
-int chain_call_args(Engine* E, int id, int call, int nb, const PassSpec& s, int pass_last,
-                    PassArgs<T>& a) {
+int chain_call_args(Engine* E, int id, int call, int nb, const PassSpec& s,
+                    const BatchCfg& bc, PassArgs<T>& a) {
   Engine::Inst& in = E->inst[(size_t)id];
   csipm_op* h = in.h;
   const int64_t off = in.off, n = h->U;
@@ -369,9 +387,8 @@ int chain_call_args(Engine* E, int id, int call, int nb, const PassSpec& s, int 
     default:
       return fail(CSIPM_ERR_ARG, "bad chain call");
   }
-  (void)pass_last;
-  if (s.kind == PK_LINES) a = lines_args<T>(h, s.axis, src, e, done);
-  else a = dim1_args<T>(h, src, real_in, nullptr, e, scale, done);
+  if (s.kind == PK_LINES) a = lines_args<T>(h, s.axis, src, e, done, &bc.lines[s.axis]);
+  else a = dim1_args<T>(h, src, real_in, nullptr, e, scale, done, &bc.d1);
   return CSIPM_OK;
 }
 
@@ -394,18 +411,18 @@ int build_chain(Engine* E, const std::vector<int>& ids, int call, int nb, Chain&
         size_t lds = 0;
         for (int id : mem) {
           csipm_op* h = E->inst[(size_t)id].h;
-          const PrecState& ps = h->ps<T>();
-          const int want = s.kind == PK_LINES ? kNT : ps.d1.nt;
+          const BatchCfg bc = batch_cfg<T>(h, (int)ids.size());
+          const int want = s.kind == PK_LINES ? kNT : bc.d1.nt;
           if (want != nt) continue;
           PassArgs<T> a{};
-          CHK(chain_call_args<T>(E, id, call, nb, s, p == np - 1, a));
+          CHK(chain_call_args<T>(E, id, call, nb, s, bc, a));
           tab.push_back(a);
           if (s.kind == PK_LINES) {
-            grids.push_back(ps.lines[s.axis].grid);
-            lds = std::max(lds, ps.lines[s.axis].lds);
+            grids.push_back(bc.lines[s.axis].grid);
+            lds = std::max(lds, bc.lines[s.axis].lds);
           } else {
-            grids.push_back(ps.d1.grid);
-            lds = std::max(lds, ps.d1.lds);
+            grids.push_back(bc.d1.grid);
+            lds = std::max(lds, bc.d1.lds);
           }
         }
         if (tab.empty()) continue;
@@ -414,7 +431,7 @@ int build_chain(Engine* E, const std::vector<int>& ids, int call, int nb, Chain&
         sub.pass = p;
         for (int id : mem) {
           csipm_op* hm = E->inst[(size_t)id].h;
-          const int want = s.kind == PK_LINES ? kNT : hm->ps<T>().d1.nt;
+          const int want = s.kind == PK_LINES ? kNT : batch_cfg<T>(hm, (int)ids.size()).d1.nt;
           if (want == nt) sub.ids.push_back(id);
         }
         sub.nt = nt;
@@ -446,11 +463,28 @@ int build_chain(Engine* E, const std::vector<int>& ids, int call, int nb, Chain&
   return CSIPM_OK;
 }
 
-// final-pass partial count of the CG apply of instance h (precision T)
+// final-pass partial count of the CG apply of instance h (precision T) in a
+// chain over `members` instances
 template <typename T>
-int cg_final_grid(csipm_op* h) {
-  if (h->dim == 1) return h->ps<T>().d1.grid;
-  return h->ps<T>().lines[h->dim - 1].grid;
+int cg_final_grid(csipm_op* h, int members) {
+  const BatchCfg bc = batch_cfg<T>(h, members);
+  if (h->dim == 1) return bc.d1.grid;
+  return bc.lines[h->dim - 1].grid;
+}
+
+// BInst.gcg of the active instances for a CG chain over all of them
+template <typename TC>
+int update_gcg(Engine* E) {
+  const int m = (int)E->active.size();
+  for (int id : E->active) {
+    Engine::Inst& in = E->inst[(size_t)id];
+    const int g = cg_final_grid<TC>(in.h, m);
+    if (g > in.len / kSlotAlign * kPartPerAlign)
+      return fail(CSIPM_ERR_UNSUPPORTED, "partial sums of instance exceed its share");
+    HIPCHK(hipMemcpyAsync(&E->info[id].gcg, &g, sizeof(int), hipMemcpyHostToDevice, E->stream));
+    HIPCHK(hipStreamSynchronize(E->stream));
+  }
+  return CSIPM_OK;
 }
 
 // vector segment + id list of `ids` into one device allocation
@@ -491,6 +525,7 @@ int engine_tables(Engine* E, int nb, bool ipm) {
   E->release_tables();
   CHK(upload_vseg(E, E->active, &E->seg_mem, &E->vseg, &E->ids, &E->vgrid));
   CHK(build_chain<TC>(E, E->active, CALL_CGA, nb, E->chains[CALL_CGA]));
+  CHK(update_gcg<TC>(E));
   if (ipm) {
     CHK(build_chain<double>(E, E->active, CALL_QB, nb, E->chains[CALL_QB]));
     if (E->mixed) {
@@ -529,7 +564,7 @@ int engine_admit(Engine* E, csipm_op* h, const double* z0, double lam, double* b
   in.beta_out = beta_out;
   // partial-sum capacity: 8 quantities per vector block, the CG apply's final pass
   const int64_t pcap = len / kSlotAlign * kPartPerAlign;
-  const int gcg = E->tc_size == 8 ? cg_final_grid<double>(h) : cg_final_grid<float>(h);
+  const int gcg = E->tc_size == 8 ? cg_final_grid<double>(h, 1) : cg_final_grid<float>(h, 1);
   if (8 * (int64_t)vblocks(h->U) > pcap || gcg > pcap)
     return fail(CSIPM_ERR_UNSUPPORTED, "partial sums of instance exceed its share");
   BInst bi{};
