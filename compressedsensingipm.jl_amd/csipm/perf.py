@@ -2,6 +2,7 @@
 
     python compressedsensingipm.jl_amd/csipm/perf.py --dims 255 255 255 --dtype f32
     python compressedsensingipm.jl_amd/csipm/perf.py --copy-mib 1024   (copy calibration)
+    python compressedsensingipm.jl_amd/csipm/perf.py --batch 64 --workload config1 --mixed
 
 Prints one JSON line with the per-pass HIP-event times (csipm_kkt_apply_timed),
 the algorithmic bytes of each pass and the achieved fraction of 8 TB/s.  The
@@ -32,6 +33,10 @@ def main():
                          "vectors); beta: csipm_kkt_apply on reference-ordered vectors; "
                          "reduced: the (M^T M + D) apply of the mixed-precision CG")
     ap.add_argument("--mixed", action="store_true", help="FP32 operator of a mixed handle")
+    ap.add_argument("--batch", type=int, default=0,
+                    help="time the batch engine's CG apply over this many instances instead")
+    ap.add_argument("--workload", default="", help="with --batch: config1 instances "
+                    "(else batch_instance problems of --dims)")
     args = ap.parse_args()
 
     import numpy as np
@@ -50,6 +55,8 @@ def main():
     from csipm import _lib
     from csipm.fft_operator import FFTOperator, _ptr
 
+    if args.batch:
+        return batch_main(args)
     dims = tuple(args.dims)
     N = int(np.prod(dims))
     rng = np.random.default_rng(0)
@@ -88,6 +95,37 @@ def main():
            "k_apply_GB/s": round(tot_b / (tot_ms * 1e-3) / 1e9, 1),
            "k_apply_frac": round(tot_b / (tot_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
            "passes": passes}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def batch_main(args):
+    """Per-launch times of the batch engine's CG operator apply."""
+    import csipm.workloads as wl
+    from csipm.batch import BatchSolver, slots_of
+    from csipm.fft_operator import FFTOperator
+    if args.workload == "config1":
+        probs = [wl.config1(instance=i + 1) for i in range(args.batch)]
+    else:
+        probs = [wl.batch_instance(tuple(args.dims), seed=i + 1) for i in range(args.batch)]
+    mixed = args.mixed or args.dtype == "f32"
+    ops = [FFTOperator(p.N, len(p.dims), p.dims, p.missing, mixed=mixed) for p in probs]
+    bs = BatchSolver(sum(slots_of(o) for o in ops), max_instances=len(ops), mixed=mixed)
+    for p, o in zip(probs, ops):
+        bs.admit(o, p.z0, p.lam)
+    bs.apply_timed(3)
+    launches = bs.apply_timed(args.reps)
+    tot_ms = sum(t[1] for t in launches)
+    tot_b = sum(t[2] for t in launches)
+    out = {"batch": args.batch, "workload": args.workload or f"batch_instance{tuple(args.dims)}",
+           "precision": "mixed" if mixed else "f64", "reps": args.reps,
+           "apply_ms": round(tot_ms, 5), "bytes": int(tot_b),
+           "apply_GB/s": round(tot_b / (tot_ms * 1e-3) / 1e9, 1),
+           "launches": {n: {"ms": round(ms, 5), "bytes": int(b),
+                            "GB/s": round(b / (ms * 1e-3) / 1e9, 1),
+                            "frac": round(b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+                        for n, ms, b in launches}}
+    bs.close()
     print(json.dumps(out), flush=True)
     return 0
 

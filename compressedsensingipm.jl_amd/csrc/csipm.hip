@@ -8,6 +8,8 @@
 #include <cstdio>
 #include <cstring>
 #include <initializer_list>
+#include <mutex>
+#include <unordered_map>
 #include <string>
 #include <complex>
 #include <vector>
@@ -282,11 +284,19 @@ Epi<T> epi_store(T* out) {
   return e;
 }
 
+// raise a kernel's dynamic-LDS limit once (the attribute persists; graph
+// capture then sees no attribute calls)
 template <typename K>
 int allow_lds(K kernel, size_t bytes) {
-  if (bytes > kLdsSmall)
-    HIPCHK(hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)bytes));
+  if (bytes <= kLdsSmall) return CSIPM_OK;
+  static std::mutex mu;
+  static std::unordered_map<const void*, size_t> done;
+  std::lock_guard<std::mutex> lock(mu);
+  size_t& have = done[(const void*)kernel];
+  if (have >= bytes) return CSIPM_OK;
+  HIPCHK(hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)bytes));
+  have = bytes;
   return CSIPM_OK;
 }
 
@@ -547,9 +557,9 @@ int solo_engine(csipm_op* h, Engine** out) {
     const bool mixed = h->prec == CSIPM_MIXED;
     CHK(engine_create(&h->solo, slots_of(h), 1, (int)h->esize(), mixed, h->stream));
   }
-  h->solo->stream = h->stream;
+  h->solo->user = h->stream;
   *out = h->solo;
-  return CSIPM_OK;
+  return engine_enter(h->solo);
 }
 
 // csipm_cg_solve: the reference-layout system converted to the spectrum
@@ -586,7 +596,8 @@ int cg_solve_solo(csipm_op* h, const T* b, T* x, const T* L1, const T* L2, const
     *status = cs.status;
   } while (false);
   engine_drop(E, id);
-  return rc;
+  if (rc) return rc;
+  return engine_leave(E);
 }
 
 // csipm_solve: the interior-point driver on a batch of one
@@ -606,7 +617,7 @@ int solve_solo(csipm_op* h, const double* z0, double lam, double tol, int64_t ma
   }
   CHK(engine_harvest(E, id, info));
   info[6] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-  return CSIPM_OK;
+  return engine_leave(E);
 }
 
 }  // namespace
@@ -1121,6 +1132,7 @@ int csipm_batch_admit(csipm_batch_t b, csipm_op_t h, const double* z0, double la
   if (!b || !h || !z0 || !beta_out || !id) return fail(CSIPM_ERR_ARG, "null argument");
   if (h->prec != b->prec) return fail(CSIPM_ERR_ARG, "handle precision differs from the batch");
   if (!(lambda >= 0)) return fail(CSIPM_ERR_ARG, "bad lambda");
+  CHK(engine_enter(b->E));
   return engine_admit(b->E, h, z0, lambda, beta_out, cg_itmax, id);
 }
 
@@ -1129,8 +1141,10 @@ int csipm_batch_step(csipm_batch_t b, double tol, int64_t max_iter, double cg_rt
   if (!b || !running) return fail(CSIPM_ERR_ARG, "null argument");
   if (!(tol > 0) || max_iter < 0) return fail(CSIPM_ERR_ARG, "bad tol / max_iter");
   const int mi = (int)std::min<int64_t>(max_iter, 1 << 30);
-  return BATCH_TC(b, engine_step<double>(b->E, tol, mi, cg_rtol, running),
-                  engine_step<float>(b->E, tol, mi, cg_rtol, running));
+  CHK(engine_enter(b->E));
+  CHK(BATCH_TC(b, engine_step<double>(b->E, tol, mi, cg_rtol, running),
+               engine_step<float>(b->E, tol, mi, cg_rtol, running)));
+  return engine_leave(b->E);
 }
 
 int csipm_batch_poll(csipm_batch_t b, int* ids, int cap, int* n) {
@@ -1154,7 +1168,9 @@ int csipm_batch_harvest(csipm_batch_t b, int id, double* info) {
   Engine* E = b->E;
   if (id < 0 || id >= E->maxinst || !E->inst[(size_t)id].h || E->inst[(size_t)id].fresh)
     return fail(CSIPM_ERR_ARG, "no admitted instance %d", id);
-  return engine_harvest(E, id, info);
+  CHK(engine_enter(E));
+  CHK(engine_harvest(E, id, info));
+  return engine_leave(E);
 }
 
 int csipm_batch_solve(csipm_batch_t b, int n, const csipm_op_t* ops, const double* const* z0,
@@ -1180,7 +1196,7 @@ int csipm_batch_solve(csipm_batch_t b, int n, const csipm_op_t* ops, const doubl
     if (running == 0) break;
   }
   for (int i = 0; i < n; ++i) CHK(engine_harvest(b->E, ids[(size_t)i], info + 8 * (size_t)i));
-  return CSIPM_OK;
+  return engine_leave(b->E);
 }
 
 int csipm_batch_apply_timed(csipm_batch_t b, int64_t nrep, double* pass_ms,
@@ -1188,6 +1204,7 @@ int csipm_batch_apply_timed(csipm_batch_t b, int64_t nrep, double* pass_ms,
   if (!b || !pass_ms || !pass_bytes || !nlaunch || nrep < 1)
     return fail(CSIPM_ERR_ARG, "bad argument");
   b->names.assign((size_t)std::max(cap, 0), std::string());
+  CHK(engine_enter(b->E));
   return BATCH_TC(b,
                   engine_apply_timed<double>(b->E, nrep, pass_ms, pass_bytes, b->names.data(),
                                              cap, nlaunch),

@@ -183,7 +183,19 @@ enum { CALL_QB = 0, CALL_REF, CALL_CGA, CALL_CGR, CALL_MTZ, CALL_N };
 struct Engine {
   int tc_size = 8;          // CG / operator precision of the CG
   bool mixed = false;       // reduced system, FP32 CG, FP64 refinement
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;   // the engine's own stream (all its work)
+  hipStream_t user = nullptr;     // the caller's stream, ordered around every call
+  hipEvent_t ev = nullptr;
+  bool use_graphs = true;         // CG chunks replayed as captured HIP graphs
+  // captured CG chunks: [0] four plain iterations, [1] three plain + one with
+  // the reliable update; valid for (graph_nb, graph_rel, graph_version)
+  hipGraphExec_t graph[2] = {nullptr, nullptr};
+  int graph_nb = 0, graph_rel = -1;
+  int64_t graph_version = -1;
+  int64_t tables_version = 0;
+  std::vector<int64_t> table_sig;  // (id, handle, offset) of the set the tables were built for
+  int table_nb = 0;
+  bool table_ipm = false;
   int64_t cap = 0;          // pool slots
   int maxinst = 0;
   struct Inst {
@@ -198,6 +210,7 @@ struct Engine {
   std::vector<Inst> inst;
   std::vector<std::pair<int64_t, int64_t>> freelist;   // (offset, length)
   // device state
+  std::vector<BInst> info_h;   // host mirror of info
   BInst* info = nullptr;
   BFlags* flags = nullptr;
   BFlags* flags_h = nullptr;
@@ -229,8 +242,19 @@ struct Engine {
     if (seg_mem) (void)hipFree(seg_mem);
     seg_mem = nullptr;
   }
+  void release_graphs() {
+    for (auto& g : graph)
+      if (g) {
+        (void)hipGraphExecDestroy(g);
+        g = nullptr;
+      }
+    graph_version = -1;
+  }
   ~Engine() {
+    release_graphs();
     release_tables();
+    if (stream) (void)hipStreamDestroy(stream);
+    if (ev) (void)hipEventDestroy(ev);
     for (void* p : {(void*)info, (void*)flags, (void*)cg, (void*)scal, (void*)part, (void*)valid,
                     (void*)Wd, Wc})
       if (p) (void)hipFree(p);
@@ -253,10 +277,12 @@ int engine_create(Engine** out, int64_t cap_slots, int maxinst, int tc_size, boo
   if (!E) return fail(CSIPM_ERR_ALLOC, "out of host memory");
   E->tc_size = tc_size;
   E->mixed = mixed;
-  E->stream = st;
+  E->user = st;
+  E->use_graphs = std::getenv("CSIPM_NO_GRAPH") == nullptr;
   E->cap = (cap_slots + kSlotAlign - 1) / kSlotAlign * kSlotAlign;
   E->maxinst = maxinst;
   E->inst.resize((size_t)maxinst);
+  E->info_h.assign((size_t)maxinst, BInst{});
   E->freelist.push_back({0, E->cap});
   auto bad = [&](int rc) { delete E; return rc; };
 #define ECHK(expr)                                                                       \
@@ -265,6 +291,8 @@ int engine_create(Engine** out, int64_t cap_slots, int maxinst, int tc_size, boo
     if (e_ != hipSuccess)                                                                \
       return bad(fail(CSIPM_ERR_ALLOC, "%s failed: %s", #expr, hipGetErrorString(e_))); \
   } while (0)
+  ECHK(hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking));
+  ECHK(hipEventCreateWithFlags(&E->ev, hipEventDisableTiming));
   ECHK(hipMalloc(&E->info, sizeof(BInst) * (size_t)maxinst));
   ECHK(hipMalloc(&E->flags, sizeof(BFlags) * (size_t)maxinst));
   ECHK(hipHostMalloc(&E->flags_h, sizeof(BFlags) * (size_t)maxinst, hipHostMallocDefault));
@@ -277,6 +305,18 @@ int engine_create(Engine** out, int64_t cap_slots, int maxinst, int tc_size, boo
   ECHK(hipMalloc(&E->Wc, (size_t)tc_size * (size_t)E->cap * (C_NV1 + 2 * C_NV2)));
 #undef ECHK
   *out = E;
+  return CSIPM_OK;
+}
+
+// order the engine's stream after the caller's work, and the caller's after ours
+int engine_enter(Engine* E) {
+  HIPCHK(hipEventRecord(E->ev, E->user));
+  HIPCHK(hipStreamWaitEvent(E->stream, E->ev, 0));
+  return CSIPM_OK;
+}
+int engine_leave(Engine* E) {
+  HIPCHK(hipEventRecord(E->ev, E->stream));
+  HIPCHK(hipStreamWaitEvent(E->user, E->ev, 0));
   return CSIPM_OK;
 }
 
@@ -481,9 +521,11 @@ int update_gcg(Engine* E) {
     const int g = cg_final_grid<TC>(in.h, m);
     if (g > in.len / kSlotAlign * kPartPerAlign)
       return fail(CSIPM_ERR_UNSUPPORTED, "partial sums of instance exceed its share");
-    HIPCHK(hipMemcpyAsync(&E->info[id].gcg, &g, sizeof(int), hipMemcpyHostToDevice, E->stream));
-    HIPCHK(hipStreamSynchronize(E->stream));
+    E->info_h[(size_t)id].gcg = g;
   }
+  HIPCHK(hipMemcpyAsync(E->info, E->info_h.data(), sizeof(BInst) * E->info_h.size(),
+                        hipMemcpyHostToDevice, E->stream));
+  HIPCHK(hipStreamSynchronize(E->stream));
   return CSIPM_OK;
 }
 
@@ -521,19 +563,35 @@ int upload_vseg(Engine* E, const std::vector<int>& ids, int** mem, BSeg* seg, co
 // interior-point iteration (ipm = false: only the CG apply, for a plain CG)
 template <typename TC>
 int engine_tables(Engine* E, int nb, bool ipm) {
+  std::vector<int64_t> sig;
+  for (int id : E->active) {
+    sig.push_back(id);
+    sig.push_back((int64_t)(intptr_t)E->inst[(size_t)id].h);
+    sig.push_back(E->inst[(size_t)id].off);
+  }
+  if (E->tables_dirty && sig == E->table_sig && nb == E->table_nb && ipm == E->table_ipm &&
+      E->seg_mem) {
+    // the same instances in the same slots: the tables stand (the instance
+    // records were rewritten by the admissions)
+    CHK(update_gcg<TC>(E));
+    E->tables_dirty = false;
+  }
   if (!E->tables_dirty) return CSIPM_OK;
   E->release_tables();
+  E->table_sig.clear();
   CHK(upload_vseg(E, E->active, &E->seg_mem, &E->vseg, &E->ids, &E->vgrid));
   CHK(build_chain<TC>(E, E->active, CALL_CGA, nb, E->chains[CALL_CGA]));
+  CHK(build_chain<TC>(E, E->active, CALL_CGR, nb, E->chains[CALL_CGR]));
   CHK(update_gcg<TC>(E));
   if (ipm) {
     CHK(build_chain<double>(E, E->active, CALL_QB, nb, E->chains[CALL_QB]));
-    if (E->mixed) {
-      CHK(build_chain<double>(E, E->active, CALL_REF, nb, E->chains[CALL_REF]));
-      CHK(build_chain<TC>(E, E->active, CALL_CGR, nb, E->chains[CALL_CGR]));
-    }
+    if (E->mixed) CHK(build_chain<double>(E, E->active, CALL_REF, nb, E->chains[CALL_REF]));
   }
   E->tables_dirty = false;
+  E->table_sig = sig;
+  E->table_nb = nb;
+  E->table_ipm = ipm;
+  E->tables_version += 1;
   return CSIPM_OK;
 }
 
@@ -576,7 +634,9 @@ int engine_admit(Engine* E, csipm_op* h, const double* z0, double lam, double* b
   bi.gcg = gcg;
   bi.itmax = (int)std::min<int64_t>(cg_itmax > 0 ? cg_itmax : 4 * h->N, 1 << 30);
   BFlags fl{};
-  HIPCHK(hipMemcpyAsync(E->info + k, &bi, sizeof(BInst), hipMemcpyHostToDevice, E->stream));
+  E->info_h[(size_t)k] = bi;
+  HIPCHK(hipMemcpyAsync(E->info + k, &E->info_h[(size_t)k], sizeof(BInst), hipMemcpyHostToDevice,
+                        E->stream));
   HIPCHK(hipMemcpyAsync(E->flags + k, &fl, sizeof(BFlags), hipMemcpyHostToDevice, E->stream));
   HIPCHK(hipMemcpyAsync(E->valid + off / 32, h->valid, sizeof(uint32_t) * (size_t)((h->U + 31) / 32),
                         hipMemcpyDeviceToDevice, E->stream));
@@ -650,11 +710,76 @@ int engine_init_fresh(Engine* E) {
 // preconditioner; nb = 1: reduced system with D in C_L1, P in C_P11).
 // reliable > 0: true residual and restart every `reliable` iterations.
 // ---------------------------------------------------------------------------
+// one CG iteration (reliable: with the true-residual restart after it)
+template <typename TC>
+int cg_iteration(Engine* E, int nb, bool reliable) {
+  const int na = (int)E->active.size();
+  TC *b = E->C2<TC>(C2_RHS), *x = E->C2<TC>(C2_D), *r = E->C2<TC>(C2_R);
+  TC *z = E->C2<TC>(C2_Z), *p = E->C2<TC>(C2_P), *q = E->C2<TC>(C2_Q);
+  TC *P11 = E->C<TC>(C_P11), *P12 = E->C<TC>(C_P12), *P22 = E->C<TC>(C_P22);
+  hipStream_t st = E->stream;
+  CHK(run_chain<TC>(E->chains[CALL_CGA], st));
+  hipLaunchKernelGGL(bk_cg_alpha, dim3(na), dim3(kNT), 0, st, E->ids, E->info, E->cg, E->part);
+  hipLaunchKernelGGL((bk_cg_update<TC>), dim3(E->vgrid), dim3(kNT), 0, st, E->vseg, E->info,
+                     E->cg, x, r, z, p, q, P11, P12, P22, nb, E->part);
+  hipLaunchKernelGGL(bk_cg_gamma, dim3(na), dim3(kNT), 0, st, E->vseg, E->ids, E->info, E->cg,
+                     E->part);
+  if (reliable) {
+    CHK(run_chain<TC>(E->chains[CALL_CGR], st));
+    hipLaunchKernelGGL((bk_cg_residual<TC>), dim3(E->vgrid), dim3(kNT), 0, st, E->vseg, E->info,
+                       E->cg, b, q, r, z, P11, P12, P22, nb, E->part);
+    hipLaunchKernelGGL(bk_cg_regamma, dim3(na), dim3(kNT), 0, st, E->vseg, E->ids, E->info,
+                       E->cg, E->part);
+  }
+  hipLaunchKernelGGL((bk_cg_pupdate<TC>), dim3(E->vgrid), dim3(kNT), 0, st, E->vseg, E->info,
+                     E->cg, p, z, nb);
+  HIPCHK(hipGetLastError());
+  return CSIPM_OK;
+}
+
+constexpr int kCgChunk = 4;   // CG iterations between host checks (one graph)
+
+// capture the two CG chunks for the current tables
+template <typename TC>
+int capture_cg_graphs(Engine* E, int nb, int reliable) {
+  E->release_graphs();
+  for (int g = 0; g < (reliable > 0 ? 2 : 1); ++g) {
+    hipGraph_t graph = nullptr;
+    HIPCHK(hipStreamBeginCapture(E->stream, hipStreamCaptureModeRelaxed));
+    int rc = CSIPM_OK;
+    for (int k = 0; k < kCgChunk && rc == CSIPM_OK; ++k)
+      rc = cg_iteration<TC>(E, nb, g == 1 && k == kCgChunk - 1);
+    const hipError_t ec = hipStreamEndCapture(E->stream, &graph);
+    if (rc) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return rc;
+    }
+    if (ec != hipSuccess)
+      return fail(CSIPM_ERR_HIP, "CG graph capture failed: %s", hipGetErrorString(ec));
+    const hipError_t ei = hipGraphInstantiate(&E->graph[g], graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess)
+      return fail(CSIPM_ERR_HIP, "CG graph instantiation failed: %s", hipGetErrorString(ei));
+  }
+  E->graph_nb = nb;
+  E->graph_rel = reliable;
+  E->graph_version = E->tables_version;
+  return CSIPM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// batched preconditioned CG on the engine's CG vectors: b = C2_RHS,
+// x = C2_D, diagonals C_L1.. (nb = 2: condensed K with the block
+// preconditioner; nb = 1: reduced system with D in C_L1, P in C_P11).
+// reliable > 0 (a multiple of kCgChunk): true residual and restart every
+// `reliable` iterations.  The host checks the instances' states every
+// kCgChunk iterations; the chunks in between are replayed HIP graphs.
+// ---------------------------------------------------------------------------
 template <typename TC>
 int engine_cg(Engine* E, int nb, double atol, double rtol, int reliable) {
   const int na = (int)E->active.size();
   TC *b = E->C2<TC>(C2_RHS), *x = E->C2<TC>(C2_D), *r = E->C2<TC>(C2_R);
-  TC *z = E->C2<TC>(C2_Z), *p = E->C2<TC>(C2_P), *q = E->C2<TC>(C2_Q);
+  TC *z = E->C2<TC>(C2_Z), *p = E->C2<TC>(C2_P);
   TC *P11 = E->C<TC>(C_P11), *P12 = E->C<TC>(C_P12), *P22 = E->C<TC>(C_P22);
   hipStream_t st = E->stream;
   hipLaunchKernelGGL((bk_cg_init<TC>), dim3(E->vgrid), dim3(kNT), 0, st, E->vseg, E->info,
@@ -662,32 +787,24 @@ int engine_cg(Engine* E, int nb, double atol, double rtol, int reliable) {
   hipLaunchKernelGGL(bk_cg_start, dim3(na), dim3(kNT), 0, st, E->vseg, E->ids, E->info, E->flags,
                      E->cg, E->part, atol, rtol);
   HIPCHK(hipGetLastError());
-  const int check_every = 4;
-  for (int64_t it = 0;; ++it) {
-    if (it % check_every == 0) {
-      HIPCHK(hipMemcpyAsync(E->cg_h, E->cg, sizeof(CgState) * (size_t)E->maxinst,
-                            hipMemcpyDeviceToHost, st));
-      HIPCHK(hipStreamSynchronize(st));
-      bool all = true;
-      for (int id : E->active) all = all && E->cg_h[id].done;
-      if (all) break;
+  bool graphs = E->use_graphs;
+  if (graphs && (E->graph_version != E->tables_version || E->graph_nb != nb ||
+                 E->graph_rel != reliable))
+    CHK(capture_cg_graphs<TC>(E, nb, reliable));
+  for (int64_t chunk = 0;; ++chunk) {
+    HIPCHK(hipMemcpyAsync(E->cg_h, E->cg, sizeof(CgState) * (size_t)E->maxinst,
+                          hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    bool all = true;
+    for (int id : E->active) all = all && E->cg_h[id].done;
+    if (all) break;
+    const bool rel = reliable > 0 && ((chunk + 1) * kCgChunk) % reliable == 0;
+    if (graphs) {
+      HIPCHK(hipGraphLaunch(E->graph[rel ? 1 : 0], st));
+    } else {
+      for (int k = 0; k < kCgChunk; ++k)
+        CHK(cg_iteration<TC>(E, nb, rel && k == kCgChunk - 1));
     }
-    CHK(run_chain<TC>(E->chains[CALL_CGA], st));
-    hipLaunchKernelGGL(bk_cg_alpha, dim3(na), dim3(kNT), 0, st, E->ids, E->info, E->cg, E->part);
-    hipLaunchKernelGGL((bk_cg_update<TC>), dim3(E->vgrid), dim3(kNT), 0, st, E->vseg, E->info,
-                       E->cg, x, r, z, p, q, P11, P12, P22, nb, E->part);
-    hipLaunchKernelGGL(bk_cg_gamma, dim3(na), dim3(kNT), 0, st, E->vseg, E->ids, E->info, E->cg,
-                       E->part);
-    if (reliable > 0 && (it + 1) % reliable == 0) {
-      CHK(run_chain<TC>(E->chains[CALL_CGR], st));
-      hipLaunchKernelGGL((bk_cg_residual<TC>), dim3(E->vgrid), dim3(kNT), 0, st, E->vseg,
-                         E->info, E->cg, b, q, r, z, P11, P12, P22, nb, E->part);
-      hipLaunchKernelGGL(bk_cg_regamma, dim3(na), dim3(kNT), 0, st, E->vseg, E->ids, E->info,
-                         E->cg, E->part);
-    }
-    hipLaunchKernelGGL((bk_cg_pupdate<TC>), dim3(E->vgrid), dim3(kNT), 0, st, E->vseg, E->info,
-                       E->cg, p, z, nb);
-    HIPCHK(hipGetLastError());
   }
   hipLaunchKernelGGL(bk_cg_account, dim3((na + 255) / 256), dim3(256), 0, st, na, E->ids, E->cg,
                      E->flags);
@@ -699,28 +816,34 @@ int engine_cg(Engine* E, int nb, double atol, double rtol, int reliable) {
 // one Newton solve.  FP64: CG on the condensed system (the reference's K),
 // direction in C2_D.  Mixed: the c-block is eliminated in FP64
 // (dc = (h_c - L2 db) / L1 leaves (M^T M + D) db = h_b - (L2/L1) h_c with
-// D = L1 - L2^2/L1 = 4 S1 S2 / (S1 + S2) > 0), the reduced system is solved
-// by FP32 CG with the diagonal preconditioner 1/(1 + D), then one FP64
-// refinement step; direction in V2_DD (FP64).  In FP32 the condensed K itself
+// D = L1 - L2^2/L1 = 4 S1 S2 / (S1 + S2) > 0) and the reduced system is
+// solved by FP32 CG with the diagonal preconditioner 1/(1 + D); the final
+// direction of each step (the corrector, refine = true) gets one FP64
+// refinement step: residual in FP64, correction by FP32 CG to kRefineRtol of
+// that residual.  Direction in V2_DD (FP64).  In FP32 the condensed K itself
 // is unusable near the solution: L1 and |L2| agree to more than 24 bits there.
 // ---------------------------------------------------------------------------
+constexpr double kRefineRtol = 1e-2;
+
 template <typename TC>
-int engine_newton(Engine* E, double rtol, int reliable) {
+int engine_newton(Engine* E, double rtol, int reliable, bool refine) {
   if (!E->mixed) return engine_cg<TC>(E, 2, 0.0, rtol, 0);
   hipStream_t st = E->stream;
   const BSeg& sg = E->vseg;
   CHK(engine_cg<TC>(E, 1, 0.0, rtol, reliable));
   hipLaunchKernelGGL((bk_convert<TC, double>), dim3(E->vgrid), dim3(kNT), 0, st, sg, E->info,
                      E->flags, E->C2<TC>(C2_D), E->W2(V2_DD), 1);
-  hipLaunchKernelGGL((bk_convert<double, double>), dim3(E->vgrid), dim3(kNT), 0, st, sg, E->info,
-                     E->flags, E->W2(V2_RHSD), E->W2(V2_TMP), 1);
-  CHK(run_chain<double>(E->chains[CALL_REF], st));
-  hipLaunchKernelGGL((bk_convert<double, TC>), dim3(E->vgrid), dim3(kNT), 0, st, sg, E->info,
-                     E->flags, E->W2(V2_TMP), E->C2<TC>(C2_RHS), 1);
-  HIPCHK(hipGetLastError());
-  CHK(engine_cg<TC>(E, 1, 0.0, rtol, reliable));
-  hipLaunchKernelGGL((bk_widen_add<TC>), dim3(E->vgrid), dim3(kNT), 0, st, sg, E->info, E->flags,
-                     E->W2(V2_DD), E->C2<TC>(C2_D));
+  if (refine) {
+    hipLaunchKernelGGL((bk_convert<double, double>), dim3(E->vgrid), dim3(kNT), 0, st, sg,
+                       E->info, E->flags, E->W2(V2_RHSD), E->W2(V2_TMP), 1);
+    CHK(run_chain<double>(E->chains[CALL_REF], st));
+    hipLaunchKernelGGL((bk_convert<double, TC>), dim3(E->vgrid), dim3(kNT), 0, st, sg, E->info,
+                       E->flags, E->W2(V2_TMP), E->C2<TC>(C2_RHS), 1);
+    HIPCHK(hipGetLastError());
+    CHK(engine_cg<TC>(E, 1, 0.0, kRefineRtol, reliable));
+    hipLaunchKernelGGL((bk_widen_add<TC>), dim3(E->vgrid), dim3(kNT), 0, st, sg, E->info,
+                       E->flags, E->W2(V2_DD), E->C2<TC>(C2_D));
+  }
   hipLaunchKernelGGL(bk_red_back, dim3(E->vgrid), dim3(kNT), 0, st, sg, E->info, E->flags,
                      E->W2(V2_DD), E->W2(V2_RHSD), E->W(V_L1D), E->W(V_L2D), E->valid);
   HIPCHK(hipGetLastError());
@@ -745,7 +868,7 @@ int engine_step(Engine* E, double tol, int max_iter, double cg_rtol, int* runnin
   const int g = E->vgrid;
   const double tau = 0.995;
   const double rtol = E->mixed ? std::max(cg_rtol, 1e-5) : cg_rtol;
-  const int reliable = E->mixed ? 25 : 0;
+  const int reliable = E->mixed ? 6 * kCgChunk : 0;
   double *b = E->W(V_B), *beta = E->W(V_BETA), *c = E->W(V_C), *z1 = E->W(V_Z1);
   double *z2 = E->W(V_Z2), *Qb = E->W(V_QB), *s1 = E->W(V_S1), *s2 = E->W(V_S2);
   double *dt1 = E->W(V_DT1), *dt2 = E->W(V_DT2), *dz1 = E->W(V_DZ1), *dz2 = E->W(V_DZ2);
@@ -772,7 +895,7 @@ int engine_step(Engine* E, double tol, int max_iter, double cg_rtol, int* runnin
                        E->C<TC>(C_P11), E->C<TC>(C_P12), E->C<TC>(C_P22), rhs, E->valid);
   HIPCHK(hipGetLastError());
   // predictor
-  CHK(engine_newton<TC>(E, rtol, reliable));
+  CHK(engine_newton<TC>(E, rtol, reliable, false));
   auto dir = [&](int corr) -> int {
     if (E->mixed)
       hipLaunchKernelGGL((bk_ipm_dir<double>), dim3(g), dim3(kNT), 0, st, sg, E->info, E->flags,
@@ -795,7 +918,7 @@ int engine_step(Engine* E, double tol, int max_iter, double cg_rtol, int* runnin
                      beta, c, z1, z2, Qb, b, dt1, dt2, dz1, dz2, E->scal, rhs, rhsd, E->W(V_L1D),
                      E->W(V_L2D), E->valid);
   HIPCHK(hipGetLastError());
-  CHK(engine_newton<TC>(E, rtol, reliable));
+  CHK(engine_newton<TC>(E, rtol, reliable, true));
   CHK(dir(1));
   CHK(finalize(E, sg, E->ids, na, {2, 2}, 3, 0, tau, tol, max_iter));
   if (E->mixed)

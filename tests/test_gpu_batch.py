@@ -8,6 +8,9 @@
 * Continuous batching: problems admitted while others are mid-solve, and
   retired as they finish, still give the alone results.
 * The mixed-precision batch reaches the FP64 oracle objective within 1e-6.
+* The CG chunks replayed as captured HIP graphs give bit for bit the result
+  of the same launches issued one by one (CSIPM_NO_GRAPH), over the many
+  replays of a solve with host synchronisations between them.
 """
 import numpy as np
 import pytest
@@ -109,3 +112,20 @@ def test_mixed_batch_vs_oracle(cuda):
         f = ipm.objective(beta, np.abs(beta), mt(beta), b, p.lam)
         f_ref = ipm.objective(ref.beta, np.abs(ref.beta), mt(ref.beta), b, p.lam)
         assert abs(f - f_ref) <= 1e-6 * abs(f_ref), p.name
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_graphs_match_eager(cuda, monkeypatch, mixed):
+    from csipm.batch import solve_batch
+    print("graphs vs eager", "mixed" if mixed else "f64", flush=True)
+    probs = _problems()[:5]
+    ops = [_op(p, mixed) for p in probs]
+    args = ([p.z0 for p in probs], [p.lam for p in probs])
+    tol = 1e-7 if mixed else 1e-8
+    res_g = solve_batch(ops, *args, mixed=mixed, tol=tol)
+    monkeypatch.setenv("CSIPM_NO_GRAPH", "1")
+    res_e = solve_batch(ops, *args, mixed=mixed, tol=tol)
+    for p, a, b in zip(probs, res_g, res_e):
+        assert a.iterations == b.iterations and a.cg_iterations == b.cg_iterations, p.name
+        assert a.objective == b.objective, p.name
+        assert torch.equal(a.beta, b.beta), p.name
