@@ -80,6 +80,13 @@ struct PrecState {
   void* binv[3] = {nullptr, nullptr, nullptr};
   LineCfg lines[3];
   Dim1Cfg d1;
+  // long 1D lines (four-step): twiddles of the column (B) and row (A)
+  // transforms (tw[0] holds those of the full length F, chirp[0].. the outer
+  // Bluestein tables) and the launch shapes of the two pass kinds
+  void* twA = nullptr;
+  void* twB = nullptr;
+  LineCfg col;
+  Dim1Cfg row;
 };
 
 }  // namespace
@@ -94,6 +101,11 @@ struct csipm_op {
   int64_t N = 0, Nc = 0, U = 0;  // U = 2*Nc: length of the spectrum-ordered copy of beta
   FftPlan plan[3]{};             // line FFT of each axis (length M for Bluestein axes)
   bool bs[3] = {false, false, false};
+  // 1D lines longer than one workgroup holds: four-step over B rows x A
+  // columns of the length-F transform (F = N, or Bluestein's M when obs)
+  bool big = false, obs = false;
+  int bA = 0, bB = 0, bF = 0;
+  FftPlan planA{}, planB{};
   PrecState p64, p32;
   uint32_t* miss = nullptr;
   int64_t nmiss = 0;
@@ -256,6 +268,35 @@ static LineCfg line_cfg(int H1, int L, int64_t others, size_t cs, int min_grid =
   return c;
 }
 
+// four-step column pass: 2^lg adjacent columns of length B per workgroup
+static LineCfg col_cfg(int A, int B, size_t cs, int min_grid = 1024) {
+  LineCfg c{};
+  int lg = 0;
+  while ((2 << lg) * B <= kNT * kE && (size_t)(2 << lg) * B * cs <= kLdsSmall && (2 << lg) <= A)
+    ++lg;
+  while (lg > 0 && ((A + (1 << lg) - 1) >> lg) < min_grid) --lg;
+  c.lg = lg;
+  c.ntile1 = (A + (1 << lg) - 1) >> lg;
+  c.grid = c.ntile1;
+  c.lds = (size_t)(1 << lg) * B * cs;
+  return c;
+}
+
+// four-step row pass: 2^lg rows of length A per workgroup (LDS stride padded)
+static Dim1Cfg row_cfg(int A, int B, size_t cs, int min_grid = 1024) {
+  Dim1Cfg c{};
+  c.nt = kNT;
+  int lg = 0;
+  auto lds_of = [&](int l) { return (size_t)A * ((1 << l) + 1) * cs; };
+  while ((2 << lg) * A <= kNT * kE && lds_of(lg + 1) <= kLdsSmall && (2 << lg) <= B) ++lg;
+  while (lg > 0 && ((B + (1 << lg) - 1) >> lg) < min_grid) --lg;
+  c.lg = lg;
+  c.ps = (1 << lg) + 1;
+  c.grid = (B + (1 << lg) - 1) >> lg;
+  c.lds = lds_of(lg);
+  return c;
+}
+
 static Dim1Cfg dim1_cfg(int N1, int nlines, size_t cs, int min_grid = 1024) {
   Dim1Cfg c{};
   c.npairs = (nlines + 1) / 2;
@@ -384,6 +425,146 @@ int launch_dim1(csipm_op* h, const void* src, const T* real_in, T* real_out, con
   return CSIPM_OK;
 }
 
+// ---------------------------------------------------------------------------
+// long 1D lines (four-step passes)
+// ---------------------------------------------------------------------------
+template <typename T>
+PassArgs<T> big_args(csipm_op* h, bool col, const void* src, const T* real_in, T* real_out,
+                     const Epi<T>& epi, T scale, const int* done, const LineCfg* cc = nullptr,
+                     const Dim1Cfg* rc = nullptr) {
+  PassArgs<T> a = base_args<T>(h, 0);
+  const PrecState& ps = h->ps<T>();
+  a.A = h->bA;
+  a.B = h->bB;
+  a.F = h->bF;
+  a.obs = h->obs ? 1 : 0;
+  a.am = magic(h->bA);
+  a.twF = (const cpx<T>*)ps.tw[0];
+  a.bs = 0;
+  if (col) {
+    const LineCfg& c = cc ? *cc : ps.col;
+    a.plan = h->planB;
+    a.tw = (const cpx<T>*)ps.twB;
+    a.lg = c.lg;
+    a.ps = 1 << c.lg;
+  } else {
+    const Dim1Cfg& c = rc ? *rc : ps.row;
+    a.plan = h->planA;
+    a.tw = (const cpx<T>*)ps.twA;
+    a.lg = c.lg;
+    a.ps = c.ps;
+  }
+  a.beta_in = (const T*)src;
+  a.u_in = (const cpx<T>*)src;
+  a.real_in = real_in;
+  a.real_out = real_out;
+  a.epi = epi;
+  a.scale = scale;
+  a.done = done;
+  return a;
+}
+
+enum { PK_LINES = 0, PK_DIM1 = 1, PK_COL = 2, PK_ROW = 3 };
+struct PassSpec {
+  int kind, D, src, dst, fm, axis;  // fm: FWD (lines), MODE (dim1, col, row)
+};
+
+// pass sequence of a long 1D line.  which: 0 M^T M (+ epilogue), 1 M^T z from
+// real samples, 2 M beta to real samples
+enum { BIG_MTM = 0, BIG_R2C = 1, BIG_C2R = 2 };
+int big_specs(const csipm_op* h, int which, int src, int dst, PassSpec* s) {
+  const bool bs = h->obs;
+  if (which == BIG_MTM) {
+    if (!bs) {
+      s[0] = {PK_COL, 1, src, DST_BUF, C_FIRST, 0};
+      s[1] = {PK_ROW, 1, SRC_BUF, DST_BUF, R_MTM, 0};
+      s[2] = {PK_COL, 1, SRC_BUF, dst, C_LAST, 0};
+      return 3;
+    }
+    s[0] = {PK_COL, 1, src, DST_BUF, C_FIRST, 0};
+    s[1] = {PK_ROW, 1, SRC_BUF, DST_BUF, R_CONV_INV, 0};
+    s[2] = {PK_COL, 1, SRC_BUF, DST_BUF, C_MID, 0};
+    s[3] = {PK_ROW, 1, SRC_BUF, DST_BUF, R_CONV_FWD, 0};
+    s[4] = {PK_COL, 1, SRC_BUF, dst, C_LAST, 0};
+    return 5;
+  }
+  if (which == BIG_R2C) {
+    if (!bs) {
+      s[0] = {PK_ROW, 1, SRC_REAL, DST_BUF, R_R2C, 0};
+      s[1] = {PK_COL, 1, SRC_BUF, dst, C_LAST, 0};
+      return 2;
+    }
+    s[0] = {PK_COL, 1, SRC_REAL, DST_BUF, C_REAL_IN, 0};
+    s[1] = {PK_ROW, 1, SRC_BUF, DST_BUF, R_CONV_FWD, 0};
+    s[2] = {PK_COL, 1, SRC_BUF, dst, C_LAST, 0};
+    return 3;
+  }
+  if (!bs) {
+    s[0] = {PK_COL, 1, src, DST_BUF, C_FIRST, 0};
+    s[1] = {PK_ROW, 1, SRC_BUF, DST_REAL, R_C2R, 0};
+    return 2;
+  }
+  s[0] = {PK_COL, 1, src, DST_BUF, C_FIRST, 0};
+  s[1] = {PK_ROW, 1, SRC_BUF, DST_BUF, R_CONV_INV, 0};
+  s[2] = {PK_COL, 1, SRC_BUF, DST_REAL, C_MID_C2R, 0};
+  return 3;
+}
+
+// one four-step pass of a single instance
+template <typename T>
+int launch_big_pass(csipm_op* h, const PassSpec& sp, const PassArgs<T>& a) {
+  const PrecState& ps = h->ps<T>();
+  const int grid = sp.kind == PK_COL ? ps.col.grid : ps.row.grid;
+  const size_t lds = sp.kind == PK_COL ? ps.col.lds : ps.row.lds;
+#define CSIPM_BIG(kern)                                                      \
+  do {                                                                       \
+    CHK(allow_lds(kern, lds));                                               \
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(kNT), lds, h->stream, a);      \
+    HIPCHK(hipGetLastError());                                               \
+    return CSIPM_OK;                                                         \
+  } while (0)
+  if (sp.kind == PK_COL) {
+    switch (sp.fm) {
+      case C_FIRST:
+        if (sp.src == SRC_BETA) CSIPM_BIG((k_col<T, kNT, kE, SRC_BETA, DST_BUF, C_FIRST>));
+        CSIPM_BIG((k_col<T, kNT, kE, SRC_U, DST_BUF, C_FIRST>));
+      case C_REAL_IN: CSIPM_BIG((k_col<T, kNT, kE, SRC_REAL, DST_BUF, C_REAL_IN>));
+      case C_MID: CSIPM_BIG((k_col<T, kNT, kE, SRC_BUF, DST_BUF, C_MID>));
+      case C_MID_C2R: CSIPM_BIG((k_col<T, kNT, kE, SRC_BUF, DST_REAL, C_MID_C2R>));
+      case C_LAST:
+        if (sp.dst == DST_BETA) CSIPM_BIG((k_col<T, kNT, kE, SRC_BUF, DST_BETA, C_LAST>));
+        CSIPM_BIG((k_col<T, kNT, kE, SRC_BUF, DST_U, C_LAST>));
+      default: break;
+    }
+  } else {
+    switch (sp.fm) {
+      case R_MTM: CSIPM_BIG((k_row<T, kNT, kE, R_MTM>));
+      case R_C2R: CSIPM_BIG((k_row<T, kNT, kE, R_C2R>));
+      case R_R2C: CSIPM_BIG((k_row<T, kNT, kE, R_R2C>));
+      case R_CONV_INV: CSIPM_BIG((k_row<T, kNT, kE, R_CONV_INV>));
+      case R_CONV_FWD: CSIPM_BIG((k_row<T, kNT, kE, R_CONV_FWD>));
+      default: break;
+    }
+  }
+#undef CSIPM_BIG
+  return fail(CSIPM_ERR_UNSUPPORTED, "no four-step kernel for pass (%d, %d)", sp.kind, sp.fm);
+}
+
+// run a four-step pass sequence on one instance; ev: an event after every pass
+template <typename T>
+int run_big(csipm_op* h, int which, int src, int dst, const void* x, const T* real_in,
+            T* real_out, const Epi<T>& epi, T scale, const int* done, hipEvent_t* ev) {
+  PassSpec sp[8];
+  const int np = big_specs(h, which, src, dst, sp);
+  for (int k = 0; k < np; ++k) {
+    const bool col = sp[k].kind == PK_COL;
+    PassArgs<T> a = big_args<T>(h, col, x, real_in, real_out, epi, scale, done);
+    CHK(launch_big_pass<T>(h, sp[k], a));
+    if (ev) HIPCHK(hipEventRecord(ev[k], h->stream));
+  }
+  return CSIPM_OK;
+}
+
 enum { SP_BETA = 0, SP_U = 1 };
 
 // the fused M^T M (epi kind 0) or condensed-KKT apply (kind 1) on beta-layout
@@ -398,6 +579,7 @@ int apply_mtm(csipm_op* h, const T* x, const Epi<T>& epi, const int* done, hipEv
     if (ev) HIPCHK(hipEventRecord(ev[k++], h->stream));
     return CSIPM_OK;
   };
+  if (D == 1 && h->big) return run_big<T>(h, BIG_MTM, SRC, DST, x, nullptr, nullptr, epi, s, done, ev);
   if (D == 1) {
     CHK((launch_dim1<T, 1, SRC, DST, MODE_MTM>(h, x, nullptr, nullptr, epi, s, done)));
     return mark();
@@ -426,6 +608,8 @@ template <typename T, int D>
 int apply_m_perp_beta(csipm_op* h, const T* beta, T* y) {
   const T s = (T)(1.0 / std::sqrt((double)h->N));
   Epi<T> e{};
+  if (D == 1 && h->big)
+    return run_big<T>(h, BIG_C2R, SRC_BETA, DST_BUF, beta, nullptr, y, e, s, nullptr, nullptr);
   if (D == 1)
     return launch_dim1<T, 1, SRC_BETA, DST_REAL, MODE_C2R>(h, beta, nullptr, y, e, s, nullptr);
   if (D == 2) {
@@ -443,6 +627,8 @@ int apply_m_perpt_z(csipm_op* h, const T* z, T* out) {
   constexpr int DST = SP == SP_U ? DST_U : DST_BETA;
   const T s = (T)(1.0 / std::sqrt((double)h->N));
   Epi<T> e = epi_store<T>(out);
+  if (D == 1 && h->big)
+    return run_big<T>(h, BIG_R2C, SRC_REAL, DST, nullptr, z, nullptr, e, s, nullptr, nullptr);
   if (D == 1)
     return launch_dim1<T, 1, SRC_REAL, DST, MODE_R2C>(h, nullptr, z, nullptr, e, s, nullptr);
   if (D == 2) {
@@ -520,6 +706,18 @@ int kkt_pass_info(csipm_op* h, int layout, const char** names, double* bytes) {
   const double mask = h->nmiss > 0 ? std::ceil(N / 8.0) : 0.0;
   const double epi = (layout == 2 ? 3 : 6) * n * s;   // x (and z), diagonals in; y out
   const bool red = layout == 2;
+  if (h->big) {
+    const double F = 2.0 * (double)h->bF * s;   // one pass over the length-F scratch
+    int k = 0;
+    names[k] = "col_first";  bytes[k++] = n * s + F;
+    names[k] = h->obs ? "row_conv_inv" : "row_mtm";  bytes[k++] = 2 * F + (h->obs ? F : mask);
+    if (h->obs) {
+      names[k] = "col_mid";      bytes[k++] = 2 * F + mask;
+      names[k] = "row_conv_fwd"; bytes[k++] = 3 * F;
+    }
+    names[k] = red ? "col_last_red" : "col_last_kkt";  bytes[k++] = F + epi;
+    return k;
+  }
   if (h->dim == 1) {
     names[0] = red ? "dim1_mtm_red" : "dim1_mtm_kkt";
     bytes[0] = epi + mask;
@@ -540,13 +738,6 @@ int kkt_pass_info(csipm_op* h, int layout, const char** names, double* bytes) {
   names[4] = red ? "lines3_fwd_red" : "lines3_fwd_kkt";
   bytes[4] = spec + epi;
   return 5;
-}
-
-// number of workgroups of the pass that ends in beta / u space
-template <typename T>
-int final_grid(csipm_op* h) {
-  if (h->dim == 1) return h->ps<T>().d1.grid;
-  return h->ps<T>().lines[h->dim - 1].grid;
 }
 
 #include "csipm_engine.hpp"
@@ -637,6 +828,9 @@ int csipm_op_destroy(csipm_op_t h) {
     for (PrecState* ps : {&h->p64, &h->p32})
       for (void* b : {ps->tw[i], ps->chirp[i], ps->bfw[i], ps->binv[i]})
         if (b) (void)hipFree(b);
+  for (PrecState* ps : {&h->p64, &h->p32})
+    for (void* b : {ps->twA, ps->twB})
+      if (b) (void)hipFree(b);
   delete h->solo;
   void* bufs[] = {h->miss, h->valid, h->buf, h->part, h->scratch};
   for (void* b : bufs)
@@ -645,8 +839,64 @@ int csipm_op_destroy(csipm_op_t h) {
   return CSIPM_OK;
 }
 
+// 1D lines above this length run as four-step transforms
+constexpr int kBig1D = 4096;
+
+// four-step plan of a long 1D line: F = N when N factors into the supported
+// radices (A x B a balanced divisor pair, both <= 4096), else Bluestein with
+// the power-of-two F >= 2N - 1; twiddles of A, B and F, the chirp tables
+static int plan_big1d(csipm_op* h) {
+  const int N = h->n[0];
+  int F = N, A = 0, B = 0;
+  FftPlan probe{};
+  h->obs = !factorize(N, probe);
+  if (h->obs) {
+    F = 1;
+    while (F < 2 * N - 1) F <<= 1;
+  }
+  // the most balanced divisor pair with both factors <= kNT * kE
+  for (int b = 1; (int64_t)b * b <= F; ++b) {
+    if (F % b) continue;
+    const int a = F / b;
+    if (a <= kNT * kE && b <= kNT * kE) {
+      A = a;
+      B = b;
+    }
+  }
+  if (A == 0 || !factorize(A, h->planA) || !factorize(B, h->planB))
+    return fail(CSIPM_ERR_UNSUPPORTED, "no four-step plan for length %d", N);
+  h->big = true;
+  h->bA = A;
+  h->bB = B;
+  h->bF = F;
+  h->plan[0].L = F;
+  for (PrecState* ps : {&h->p64, &h->p32}) {
+    const bool want = ps == &h->p64 ? h->has64() : h->has32();
+    if (!want) continue;
+    if (ps == &h->p64) {
+      CHK(upload_twiddles<double>(A, &ps->twA));
+      CHK(upload_twiddles<double>(B, &ps->twB));
+      CHK(upload_twiddles<double>(F, &ps->tw[0]));
+      if (h->obs) CHK(upload_bluestein<double>(N, F, &ps->chirp[0], &ps->bfw[0], &ps->binv[0]));
+    } else {
+      CHK(upload_twiddles<float>(A, &ps->twA));
+      CHK(upload_twiddles<float>(B, &ps->twB));
+      CHK(upload_twiddles<float>(F, &ps->tw[0]));
+      if (h->obs) CHK(upload_bluestein<float>(N, F, &ps->chirp[0], &ps->bfw[0], &ps->binv[0]));
+    }
+  }
+  return CSIPM_OK;
+}
+
 static int set_shapes(csipm_op* h, PrecState& ps, size_t cs) {
   const int dim = h->dim;
+  if (h->big) {
+    ps.col = col_cfg(h->bA, h->bB, cs);
+    ps.row = row_cfg(h->bA, h->bB, cs);
+    if (ps.col.lds > kLdsMax || ps.row.lds > kLdsMax)
+      return fail(CSIPM_ERR_UNSUPPORTED, "four-step plan of length %d exceeds the LDS", h->n[0]);
+    return CSIPM_OK;
+  }
   const int F0 = h->plan[0].L;   // FFT length (line length, or Bluestein's M)
   ps.d1 = dim1_cfg(F0, (int)(h->N / h->n[0]), cs);
   if (ps.d1.nt * kE < F0 || ps.d1.lds > kLdsMax)
@@ -671,10 +921,12 @@ static int create_impl(csipm_op* h, int dim, const int64_t sizes[3], int rdft, i
   h->dim = dim;
   h->prec = precision;
   h->stream = (hipStream_t)stream;
+  // 1D lines up to 2^23 (four-step, Bluestein length <= 2^24); 2D / 3D axes up to 16384
+  const int64_t smax = dim == 1 ? ((int64_t)1 << 23) : 16384;
   for (int i = 0; i < dim; ++i) {
-    if (sizes[i] < 2 || sizes[i] > 16384)
-      return fail(CSIPM_ERR_SHAPE, "size %d = %lld out of range [2, 16384]", i + 1,
-                  (long long)sizes[i]);
+    if (sizes[i] < 2 || sizes[i] > smax)
+      return fail(CSIPM_ERR_SHAPE, "size %d = %lld out of range [2, %lld]", i + 1,
+                  (long long)sizes[i], (long long)smax);
     h->n[i] = (int)sizes[i];
   }
   h->g = make_map_geom(h->dim, h->n);
@@ -683,7 +935,8 @@ static int create_impl(csipm_op* h, int dim, const int64_t sizes[3], int rdft, i
   h->H1 = h->n[0] / 2 + 1;
   h->Nc = (int64_t)h->H1 * h->n[1] * h->n[2];
   h->U = 2 * h->Nc;
-  for (int i = 0; i < dim; ++i) {
+  if (dim == 1 && h->n[0] > kBig1D) CHK(plan_big1d(h));
+  for (int i = 0; i < dim && !h->big; ++i) {
     if (!plan_axis(h->n[i], h->plan[i], h->bs[i]))
       return fail(CSIPM_ERR_UNSUPPORTED, "no FFT plan for size %d", h->n[i]);
     const int F = h->plan[i].L;
@@ -731,7 +984,7 @@ static int create_impl(csipm_op* h, int dim, const int64_t sizes[3], int rdft, i
       hipLaunchKernelGGL((k_u_valid<3>), dim3(grid), dim3(kNT), 0, h->stream, h->valid, h->g, h->H1, h->Nc);
     HIPCHK(hipGetLastError());
   }
-  HIPCHK(hipMalloc(&h->buf, (size_t)h->Nc * sizeof(cpx<double>)));
+  HIPCHK(hipMalloc(&h->buf, (size_t)std::max<int64_t>(h->Nc, h->bF) * sizeof(cpx<double>)));
   int np = 4096;
   for (PrecState* ps : {&h->p64, &h->p32}) {
     np = std::max(np, ps->d1.grid);

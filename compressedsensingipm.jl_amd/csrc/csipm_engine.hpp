@@ -47,15 +47,16 @@ int from_u(csipm_op* h, const T* u, T* beta, hipStream_t st) {
 // ---------------------------------------------------------------------------
 // operator chains
 // ---------------------------------------------------------------------------
-enum { PK_LINES = 0, PK_DIM1 = 1 };
-struct PassSpec {
-  int kind, D, src, dst, fm, axis;  // fm: FWD (lines) or MODE (dim1)
-};
-
 // MTM: the fused M^T M pass sequence on u (with the epilogue in the last
 // pass); MTZ: M^T z from real samples into u
 enum { CH_MTM = 0, CH_MTZ = 1 };
-int chain_specs(int dim, int which, PassSpec* s) {
+
+// instances whose chains have the same pass sequence share launches
+int chain_key(const csipm_op* h) { return h->big ? (h->obs ? 11 : 10) : h->dim; }
+
+int chain_specs(const csipm_op* h, int which, PassSpec* s) {
+  if (h->big) return big_specs(h, which == CH_MTM ? BIG_MTM : BIG_R2C, SRC_U, DST_U, s);
+  const int dim = h->dim;
   if (which == CH_MTM) {
     if (dim == 1) { s[0] = {PK_DIM1, 1, SRC_U, DST_U, MODE_MTM, 0}; return 1; }
     if (dim == 2) {
@@ -109,6 +110,18 @@ int launch_spec(const PassSpec& s, int grid, size_t lds, hipStream_t st, const v
       CSIPM_LAUNCH((kb_lines<T, kNT, kE, 3, SRC_BUF, DST_BUF, true>));
     if (s.D == 3 && s.src == SRC_BUF && s.dst == DST_U && s.fm)
       CSIPM_LAUNCH((kb_lines<T, kNT, kE, 3, SRC_BUF, DST_U, true>));
+  } else if (s.kind == PK_COL) {
+    if (NT != kNT) return fail(CSIPM_ERR_UNSUPPORTED, "column pass with %d threads", NT);
+    if (s.fm == C_FIRST) CSIPM_LAUNCH((kb_col<T, kNT, kE, SRC_U, DST_BUF, C_FIRST>));
+    if (s.fm == C_REAL_IN) CSIPM_LAUNCH((kb_col<T, kNT, kE, SRC_REAL, DST_BUF, C_REAL_IN>));
+    if (s.fm == C_MID) CSIPM_LAUNCH((kb_col<T, kNT, kE, SRC_BUF, DST_BUF, C_MID>));
+    if (s.fm == C_LAST) CSIPM_LAUNCH((kb_col<T, kNT, kE, SRC_BUF, DST_U, C_LAST>));
+  } else if (s.kind == PK_ROW) {
+    if (NT != kNT) return fail(CSIPM_ERR_UNSUPPORTED, "row pass with %d threads", NT);
+    if (s.fm == R_MTM) CSIPM_LAUNCH((kb_row<T, kNT, kE, R_MTM>));
+    if (s.fm == R_R2C) CSIPM_LAUNCH((kb_row<T, kNT, kE, R_R2C>));
+    if (s.fm == R_CONV_INV) CSIPM_LAUNCH((kb_row<T, kNT, kE, R_CONV_INV>));
+    if (s.fm == R_CONV_FWD) CSIPM_LAUNCH((kb_row<T, kNT, kE, R_CONV_FWD>));
   } else {
     if (s.D == 1 && s.src == SRC_U && s.fm == MODE_MTM)
       CSIPM_LAUNCH((kb_dim1<T, NT, kE, 1, SRC_U, DST_U, MODE_MTM>));
@@ -359,6 +372,8 @@ int vblocks(int64_t n) {
 struct BatchCfg {
   LineCfg lines[3];
   Dim1Cfg d1;
+  LineCfg col;   // four-step passes of long 1D lines
+  Dim1Cfg row;
 };
 
 template <typename T>
@@ -366,6 +381,12 @@ BatchCfg batch_cfg(csipm_op* h, int members) {
   const int target = std::max(1, 1024 / std::max(1, members));
   const size_t cs = sizeof(cpx<T>);
   BatchCfg b{};
+  if (h->big) {
+    b.col = col_cfg(h->bA, h->bB, cs, target);
+    b.row = row_cfg(h->bA, h->bB, cs, target);
+    b.d1.nt = kNT;
+    return b;
+  }
   b.d1 = dim1_cfg(h->plan[0].L, (int)(h->N / h->n[0]), cs, target);
   if (h->dim >= 2) b.lines[1] = line_cfg(h->H1, h->plan[1].L, h->n[2], cs, target);
   if (h->dim == 3) b.lines[2] = line_cfg(h->H1, h->plan[2].L, h->n[1], cs, target);
@@ -428,7 +449,8 @@ int chain_call_args(Engine* E, int id, int call, int nb, const PassSpec& s,
       return fail(CSIPM_ERR_ARG, "bad chain call");
   }
   if (s.kind == PK_LINES) a = lines_args<T>(h, s.axis, src, e, done, &bc.lines[s.axis]);
-  else a = dim1_args<T>(h, src, real_in, nullptr, e, scale, done, &bc.d1);
+  else if (s.kind == PK_DIM1) a = dim1_args<T>(h, src, real_in, nullptr, e, scale, done, &bc.d1);
+  else a = big_args<T>(h, s.kind == PK_COL, src, real_in, nullptr, e, scale, done, &bc.col, &bc.row);
   return CSIPM_OK;
 }
 
@@ -436,13 +458,14 @@ template <typename T>
 int build_chain(Engine* E, const std::vector<int>& ids, int call, int nb, Chain& out) {
   out.release();
   out.esize = sizeof(T);
-  for (int dim = 1; dim <= 3; ++dim) {
+  for (int key : {1, 2, 3, 10, 11}) {
     std::vector<int> mem;
     for (int id : ids)
-      if (E->inst[(size_t)id].h->dim == dim) mem.push_back(id);
+      if (chain_key(E->inst[(size_t)id].h) == key) mem.push_back(id);
     if (mem.empty()) continue;
     PassSpec specs[8];
-    const int np = chain_specs(dim, call == CALL_MTZ ? CH_MTZ : CH_MTM, specs);
+    const int np = chain_specs(E->inst[(size_t)mem[0]].h, call == CALL_MTZ ? CH_MTZ : CH_MTM,
+                               specs);
     for (int p = 0; p < np; ++p) {
       const PassSpec& s = specs[p];
       for (int nt : {kNT, kNTBig}) {
@@ -452,7 +475,7 @@ int build_chain(Engine* E, const std::vector<int>& ids, int call, int nb, Chain&
         for (int id : mem) {
           csipm_op* h = E->inst[(size_t)id].h;
           const BatchCfg bc = batch_cfg<T>(h, (int)ids.size());
-          const int want = s.kind == PK_LINES ? kNT : bc.d1.nt;
+          const int want = s.kind == PK_DIM1 ? bc.d1.nt : kNT;
           if (want != nt) continue;
           PassArgs<T> a{};
           CHK(chain_call_args<T>(E, id, call, nb, s, bc, a));
@@ -460,9 +483,15 @@ int build_chain(Engine* E, const std::vector<int>& ids, int call, int nb, Chain&
           if (s.kind == PK_LINES) {
             grids.push_back(bc.lines[s.axis].grid);
             lds = std::max(lds, bc.lines[s.axis].lds);
-          } else {
+          } else if (s.kind == PK_DIM1) {
             grids.push_back(bc.d1.grid);
             lds = std::max(lds, bc.d1.lds);
+          } else if (s.kind == PK_COL) {
+            grids.push_back(bc.col.grid);
+            lds = std::max(lds, bc.col.lds);
+          } else {
+            grids.push_back(bc.row.grid);
+            lds = std::max(lds, bc.row.lds);
           }
         }
         if (tab.empty()) continue;
@@ -471,7 +500,7 @@ int build_chain(Engine* E, const std::vector<int>& ids, int call, int nb, Chain&
         sub.pass = p;
         for (int id : mem) {
           csipm_op* hm = E->inst[(size_t)id].h;
-          const int want = s.kind == PK_LINES ? kNT : batch_cfg<T>(hm, (int)ids.size()).d1.nt;
+          const int want = s.kind == PK_DIM1 ? batch_cfg<T>(hm, (int)ids.size()).d1.nt : kNT;
           if (want == nt) sub.ids.push_back(id);
         }
         sub.nt = nt;
@@ -508,6 +537,7 @@ int build_chain(Engine* E, const std::vector<int>& ids, int call, int nb, Chain&
 template <typename T>
 int cg_final_grid(csipm_op* h, int members) {
   const BatchCfg bc = batch_cfg<T>(h, members);
+  if (h->big) return bc.col.grid;
   if (h->dim == 1) return bc.d1.grid;
   return bc.lines[h->dim - 1].grid;
 }
@@ -1015,7 +1045,8 @@ int engine_apply_timed(Engine* E, int64_t nrep, double* ms, double* bytes, std::
     kkt_pass_info(E->inst[(size_t)sb.ids[0]].h, layout, nm, by);
     ms[k] = acc[(size_t)k] / (double)nrep;
     bytes[k] = b;
-    names[k] = std::to_string(sb.spec.D) + "d:" + nm[sb.pass];
+    names[k] = std::to_string(sb.spec.D) + (sb.spec.kind >= PK_COL ? "d-4step:" : "d:") +
+               nm[sb.pass];
   }
   *nsub = ns;
   return CSIPM_OK;

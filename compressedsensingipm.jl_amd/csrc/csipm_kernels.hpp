@@ -321,6 +321,12 @@ struct PassArgs {
   const cpx<T>* chirp;  // w_n = exp(-i pi n^2 / len), n < len
   const cpx<T>* bfw;    // FFT_M(b) / M of the forward kernel b = conj(w) (wrapped)
   const cpx<T>* binv;   // the same for the inverse kernel b = w
+  // long 1D lines (four-step): the length-F transform (F = len, or the
+  // Bluestein length when obs) as A columns x B rows, position col + A*row
+  int A, B, F;
+  int obs;              // outer Bluestein: chirp / bfw / binv are of length len / F / F
+  uint32_t am;          // ceil(2^32 / A)
+  const cpx<T>* twF;    // exp(-2 pi i j / F), j < F
 };
 
 // DFT of length a.len of the 2^lg lines in LDS ([element][line], stride ps),
@@ -549,6 +555,199 @@ template <typename T, int NT, int E, int D, int SRC, int DST, int MODE>
 __global__ __launch_bounds__(NT) void kb_dim1(const PassArgs<T>* __restrict__ tab, BSeg seg) {
   const int i = seg.inst[blockIdx.x];
   dim1_body<T, NT, E, D, SRC, DST, MODE>(tab[i], blockIdx.x - seg.first[i]);
+}
+
+// ===========================================================================
+// Long 1D lines: the length-F DFT in four steps over the (B rows x A columns)
+// view of the scratch spectrum (position col + A*row).  Column passes
+// transform along rows (length B, stride A, 2^lg adjacent columns per
+// workgroup), row passes along columns (length A, contiguous, 2^lg rows).
+// Direct (F = N):
+//   MTM   col FIRST (inverse, twiddle)  row MTM (inverse, mask at the time
+//         index t = row + B*col, forward, twiddle)  col LAST (forward, epilogue)
+// Bluestein (F = M >= 2N - 1, chirp w_n = exp(-i pi n^2/N)):
+//   MTM   col FIRST (x conj(w), forward, twiddle)  row CONV_INV (forward,
+//         x FFT_M(w)/M, inverse, twiddle)  col MID (inverse, x conj(w), mask,
+//         x w, forward, twiddle)  row CONV_FWD  col LAST (inverse, x w, epilogue)
+// ===========================================================================
+enum { C_FIRST = 0, C_REAL_IN = 1, C_MID = 2, C_LAST = 3, C_MID_C2R = 4 };
+enum { R_MTM = 0, R_C2R = 1, R_R2C = 2, R_CONV_INV = 3, R_CONV_FWD = 4 };
+
+template <typename T>
+__device__ __forceinline__ cpx<T> twf(const PassArgs<T>& a, int64_t j, bool inv) {
+  cpx<T> w = a.twF[j];
+  if (inv) w.y = -w.y;
+  return w;
+}
+
+template <typename T>
+__device__ __forceinline__ cpx<T> chirp_at(const PassArgs<T>& a, int64_t n, bool conj_) {
+  cpx<T> w = a.chirp[n];
+  if (conj_) w.y = -w.y;
+  return w;
+}
+
+// spectrum value k (< N) of a Hermitian line held as its half (k < H1)
+template <typename T, int SRC>
+__device__ __forceinline__ cpx<T> half_load(const PassArgs<T>& a, int64_t k) {
+  if (k < a.H1) return spec_load<1, T, SRC>(a, (int)k, 0, 0);
+  const cpx<T> v = spec_load<1, T, SRC>(a, (int)(a.len - k), 0, 0);
+  return {v.x, -v.y};
+}
+
+template <typename T, int NT, int E, int SRC, int DST, int MODE>
+__device__ __forceinline__ void col_body(const PassArgs<T>& a, int tile) {
+  extern __shared__ unsigned char smem_raw[];
+  cpx<T>* lds = reinterpret_cast<cpx<T>*>(smem_raw);
+  if (a.done && *a.done) return;
+  const int A = a.A, B = a.B, lg = a.lg, nl = 1 << lg;
+  const int64_t N = a.len;
+  const bool bs = a.obs != 0;
+  const int c0 = tile << lg;
+  const int nv = min(nl, A - c0);
+  for (int idx = threadIdx.x; idx < (B << lg); idx += NT) {
+    const int line = idx & (nl - 1), r = idx >> lg;
+    const int64_t n = c0 + line + (int64_t)A * r;
+    cpx<T> v = {(T)0, (T)0};
+    if (line < nv) {
+      if (MODE == C_FIRST) {
+        if (n < N) {
+          v = half_load<T, SRC>(a, n);
+          if (bs) v = cmul(v, chirp_at(a, n, true));
+        }
+      } else if (MODE == C_REAL_IN) {
+        if (n < N) {
+          v = {a.real_in[n] * a.scale, (T)0};
+          if (bs) v = cmul(v, chirp_at(a, n, false));
+        }
+      } else {
+        v = a.buf[n];
+      }
+    }
+    lds[r * nl + line] = v;
+  }
+  __syncthreads();
+  const bool fwd1 = MODE == C_FIRST ? bs : (MODE == C_REAL_IN ? true : (MODE == C_LAST ? !bs : false));
+  if (fwd1) lds_fft<T, E, NT, true>(lds, a.plan, lg, nl, a.tw);
+  else lds_fft<T, E, NT, false>(lds, a.plan, lg, nl, a.tw);
+  if (MODE == C_MID || MODE == C_MID_C2R) {
+    // conv -> x_n = conj(w_n) conv_n, masked real part; then (MTM) x w for the forward
+    for (int idx = threadIdx.x; idx < (B << lg); idx += NT) {
+      const int line = idx & (nl - 1), r = idx >> lg;
+      const int64_t n = c0 + line + (int64_t)A * r;
+      cpx<T>* q = lds + r * nl + line;
+      T xr = (T)0;
+      if (line < nv && n < N && !miss_bit(a.miss, n)) xr = cmul(*q, chirp_at(a, n, true)).x * a.scale;
+      if (MODE == C_MID_C2R) {
+        if (line < nv && n < N) a.real_out[n] = xr;
+      } else {
+        const cpx<T> w = n < N ? chirp_at(a, n, false) : cpx<T>{(T)0, (T)0};
+        *q = {xr * w.x, xr * w.y};
+      }
+    }
+    if (MODE == C_MID_C2R) return;
+    __syncthreads();
+    lds_fft<T, E, NT, true>(lds, a.plan, lg, nl, a.tw);
+  }
+  double acc = 0.0;
+  for (int idx = threadIdx.x; idx < (B << lg); idx += NT) {
+    const int line = idx & (nl - 1), r = idx >> lg;
+    if (line >= nv) continue;
+    const int col = c0 + line;
+    cpx<T> v = lds[r * nl + line];
+    if (MODE != C_LAST) {
+      const bool inv = MODE == C_FIRST && !bs;
+      v = cmul(v, twf(a, (int64_t)col * r, inv));
+      a.buf[col + (int64_t)A * r] = v;
+    } else {
+      const int64_t k = col + (int64_t)A * r;
+      if (k < a.H1) {
+        if (bs) v = cmul(v, chirp_at(a, k, false));
+        acc += spec_store<1, T, DST>(a, (int)k, 0, 0, v);
+      }
+    }
+  }
+  if (MODE == C_LAST && a.epi.dot_part) store_partial(a.epi.dot_part, acc, tile);
+}
+
+template <typename T, int NT, int E, int MODE>
+__device__ __forceinline__ void row_body(const PassArgs<T>& a, int tile) {
+  extern __shared__ unsigned char smem_raw[];
+  cpx<T>* lds = reinterpret_cast<cpx<T>*>(smem_raw);
+  if (a.done && *a.done) return;
+  const int A = a.A, B = a.B, lg = a.lg, nl = 1 << lg, ps = a.ps;
+  const int r0 = tile << lg;
+  const int nv = min(nl, B - r0);
+  const int tot = A << lg;
+  for (int idx = threadIdx.x; idx < tot; idx += NT) {
+    const int line = udiv(idx, A, a.am);
+    const int c = idx - line * A;
+    const int row = r0 + line;
+    cpx<T> v = {(T)0, (T)0};
+    if (line < nv) {
+      if (MODE == R_R2C) v = {a.real_in[row + (int64_t)B * c] * a.scale, (T)0};
+      else v = a.buf[c + (int64_t)A * row];
+    }
+    lds[c * ps + line] = v;
+  }
+  __syncthreads();
+  const bool fwd1 = MODE == R_R2C || MODE == R_CONV_INV || MODE == R_CONV_FWD;
+  if (fwd1) lds_fft<T, E, NT, true>(lds, a.plan, lg, ps, a.tw);
+  else lds_fft<T, E, NT, false>(lds, a.plan, lg, ps, a.tw);
+  if (MODE == R_MTM || MODE == R_C2R) {
+    for (int idx = threadIdx.x; idx < tot; idx += NT) {
+      const int line = udiv(idx, A, a.am);
+      const int c = idx - line * A;
+      const int64_t t = r0 + line + (int64_t)B * c;
+      cpx<T>* q = lds + c * ps + line;
+      const T xr = (line < nv && !miss_bit(a.miss, t)) ? q->x * a.scale : (T)0;
+      if (MODE == R_C2R) {
+        if (line < nv) a.real_out[t] = xr;
+      } else {
+        *q = {xr, (T)0};
+      }
+    }
+    if (MODE == R_C2R) return;
+    __syncthreads();
+    lds_fft<T, E, NT, true>(lds, a.plan, lg, ps, a.tw);
+  } else if (MODE == R_CONV_INV || MODE == R_CONV_FWD) {
+    const cpx<T>* Bh = MODE == R_CONV_FWD ? a.bfw : a.binv;
+    for (int idx = threadIdx.x; idx < tot; idx += NT) {
+      const int line = udiv(idx, A, a.am);
+      const int c = idx - line * A;
+      cpx<T>* q = lds + c * ps + line;
+      if (line < nv) *q = cmul(*q, Bh[r0 + line + (int64_t)B * c]);
+    }
+    __syncthreads();
+    lds_fft<T, E, NT, false>(lds, a.plan, lg, ps, a.tw);
+  }
+  const bool inv_tw = MODE == R_CONV_INV || MODE == R_CONV_FWD;
+  for (int idx = threadIdx.x; idx < tot; idx += NT) {
+    const int line = udiv(idx, A, a.am);
+    const int c = idx - line * A;
+    if (line >= nv) continue;
+    const int row = r0 + line;
+    a.buf[c + (int64_t)A * row] = cmul(lds[c * ps + line], twf(a, (int64_t)row * c, inv_tw));
+  }
+}
+
+template <typename T, int NT, int E, int SRC, int DST, int MODE>
+__global__ __launch_bounds__(NT) void k_col(PassArgs<T> a) {
+  col_body<T, NT, E, SRC, DST, MODE>(a, blockIdx.x);
+}
+template <typename T, int NT, int E, int SRC, int DST, int MODE>
+__global__ __launch_bounds__(NT) void kb_col(const PassArgs<T>* __restrict__ tab, BSeg seg) {
+  const int i = seg.inst[blockIdx.x];
+  col_body<T, NT, E, SRC, DST, MODE>(tab[i], blockIdx.x - seg.first[i]);
+}
+template <typename T, int NT, int E, int MODE>
+__global__ __launch_bounds__(NT) void k_row(PassArgs<T> a) {
+  row_body<T, NT, E, MODE>(a, blockIdx.x);
+}
+template <typename T, int NT, int E, int MODE>
+__global__ __launch_bounds__(NT) void kb_row(const PassArgs<T>* __restrict__ tab, BSeg seg) {
+  const int i = seg.inst[blockIdx.x];
+  row_body<T, NT, E, MODE>(tab[i], blockIdx.x - seg.first[i]);
 }
 
 // beta <-> u (exact: a signed permutation), one thread per half-spectrum element.

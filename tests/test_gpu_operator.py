@@ -66,9 +66,13 @@ def test_operator_golden(cuda, fname):
 
 # sizes with a prime factor above 17 run through Bluestein line transforms
 BLUESTEIN = [(127,), (4001,), (19, 23), (127, 60), (23, 19, 29), (12, 10, 37)]
+# 1D lines above 4096 run as four-step transforms: direct when N factors into
+# the supported radices, else with an outer Bluestein convolution
+FOURSTEP = [(8192,), (50000,), (65536,), (4097,), (12289,), (65521,)]
 
 
-@pytest.mark.parametrize("dims", [(1024,), (255, 255), (64, 48, 30), (51, 45, 35)] + BLUESTEIN)
+@pytest.mark.parametrize("dims", [(1024,), (255, 255), (64, 48, 30), (51, 45, 35)] + BLUESTEIN +
+                         FOURSTEP)
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
 def test_operator_masked_vs_oracle(cuda, dims, dtype):
     from csipm.fft_operator import M_perp_beta, M_perpt_M_perp_vec, M_perpt_z
@@ -120,7 +124,8 @@ def test_kkt_and_precond(cuda, dims, dtype):
     assert _rel(y.double().cpu().numpy(), want) <= 10 * tol
 
 
-@pytest.mark.parametrize("dims", [(1024,), (255, 255), (33, 30, 26), (127, 60)])
+@pytest.mark.parametrize("dims", [(1024,), (255, 255), (33, 30, 26), (127, 60), (20000,),
+                                  (12289,)])
 def test_cg_solve(cuda, dims):
     import ctypes
     from csipm import _lib

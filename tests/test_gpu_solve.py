@@ -6,7 +6,8 @@
   lambda = 0, no mask) within 1e-6.
 * Masked problems: objective within 1e-6 relative of the FP64 CPU oracle
   (solved to 1e-10 and certified by the LASSO duality gap), same support
-  above 1e-6 * max|beta|.
+  above 1e-6 * max|beta| on the entries clearly (10x) away from that
+  threshold in both solutions.
 """
 import numpy as np
 import pytest
@@ -101,7 +102,7 @@ def _support(beta):
 
 @pytest.mark.parametrize("mixed", [False, True])
 @pytest.mark.parametrize("case", ["ex1d", "ex2d", "ex3d", "cfg0", "b1d", "b2d", "b3d", "p1d",
-                                  "p2d"])
+                                  "p2d", "big1d", "bigp1d"])
 def test_solve_vs_oracle(cuda, case, mixed):
     import csipm.workloads as wl
     from csipm.model import ipm_solve
@@ -114,7 +115,9 @@ def test_solve_vs_oracle(cuda, case, mixed):
          "b2d": lambda: wl.batch_instance((63, 63), seed=12),
          "b3d": lambda: wl.batch_instance((26, 21, 15), seed=13),
          "p1d": lambda: wl.batch_instance((2003,), seed=14),
-         "p2d": lambda: wl.batch_instance((61, 53), seed=15)}[case]()
+         "p2d": lambda: wl.batch_instance((61, 53), seed=15),
+         "big1d": lambda: wl.batch_instance((20000,), seed=16),
+         "bigp1d": lambda: wl.batch_instance((12289,), seed=17)}[case]()
     b = op.m_perpt_z(p.z0, p.dims)
     mt = lambda v: op.mtm(v, p.dims, p.missing)    # noqa: E731
     ref = ipm.solve(b, p.lam, mt, ipm.IPMOptions(tol=1e-10, cg_rtol=1e-13))
@@ -129,7 +132,13 @@ def test_solve_vs_oracle(cuda, case, mixed):
     f_ref = ipm.objective(ref.beta, np.abs(ref.beta), mt(ref.beta), b, p.lam)
     assert abs(f_gpu - f_ref) <= 1e-6 * abs(f_ref)
     sg, sr = _support(beta), _support(ref.beta)
-    # entries whose magnitude sits within 10x of the threshold may flip
+    # entries within 10x of the threshold in either solution may flip with the
+    # IPM tolerance (the oracle at 1e-8 vs 1e-10 does so on big1d); the others
+    # must agree
     thr = 1e-6 * np.abs(ref.beta).max()
-    clear = (np.abs(ref.beta) > 10 * thr) | (np.abs(ref.beta) < 0.1 * thr)
-    assert np.array_equal(sg[clear], sr[clear])
+
+    def clear(v):
+        return (np.abs(v) > 10 * thr) | (np.abs(v) < 0.1 * thr)
+    both = clear(ref.beta) & clear(beta)
+    assert np.array_equal(sg[both], sr[both])
+    assert both.mean() > 0.99
