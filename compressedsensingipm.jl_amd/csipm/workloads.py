@@ -131,10 +131,11 @@ def _seed(config, instance):
 
 def _signal(rng, dims, nmodes, kmax=None):
     """Sum of nmodes random Fourier modes: integer wavevectors, N(0,1)
-    amplitudes, U[0, 2pi) phases."""
+    amplitudes, U[0, 2pi) phases.  The modes are summed by one inverse FFT of
+    the sparse spectrum sum_j a_j exp(i phi_j) delta(k - f_j) (the same values
+    as summing the cosines, to rounding, at N log N cost)."""
     dims = tuple(dims)
     d = len(dims)
-    coords = _grid_coords(dims)
     if d == 1:
         n = dims[0]
         hi = kmax if kmax is not None else n // 2 - 1
@@ -145,13 +146,13 @@ def _signal(rng, dims, nmodes, kmax=None):
         freqs = rng.integers(lo, hi + 1, size=(nmodes, d))
     amps = rng.standard_normal(nmodes)
     phases = rng.uniform(0.0, 2 * np.pi, nmodes)
-    x = np.zeros(int(np.prod(dims)))
+    # C-order array of the reversed shape: the last axis is Julia's dimension 1
+    spec = np.zeros(dims[::-1], dtype=np.complex128)
     for j in range(nmodes):
-        arg = np.zeros_like(x)
-        for ax in range(d):
-            arg += freqs[j, ax] * coords[ax] / dims[ax]
-        x += amps[j] * np.cos(2 * np.pi * arg + phases[j])
-    return x
+        idx = tuple(int(freqs[j, ax]) % dims[ax] for ax in reversed(range(d)))
+        spec[idx] += amps[j] * np.exp(1j * phases[j])
+    x = np.real(np.fft.ifftn(spec)) * spec.size
+    return x.reshape(-1)
 
 
 def _gaps_1d(rng, n, ngaps=3, width=32):
@@ -264,6 +265,26 @@ def batch_instance(dims, seed, lam=0.1):
 def config3(n1d=256, n2d=128):
     shapes = config3_shapes(n1d, n2d)
     return [batch_instance(s, _seed(3, i + 1)) for i, s in enumerate(shapes)]
+
+
+def config4_problem(shape, seed):
+    """One serving request: the generator of configs 0 / 1 / 2 at its shape
+    (1D: 12 + N/128 modes, 3 gaps of 32 + 5% dropout, lambda 0.1; 2D: 40
+    modes, r=3 discs on a 16-lattice, lambda 0.05; 3D: 200 modes, r=2 balls
+    on an 8-lattice + 1% dropout, lambda 0.05)."""
+    rng = np.random.default_rng(seed)
+    dims = tuple(shape)
+    if len(dims) == 1:
+        n = dims[0]
+        x = _signal(rng, dims, 12 + n // 128, kmax=(n - 1) // 2)
+        return _finish(dims, x, 0.01, rng, _gaps_1d(rng, n), 0.05, 0.1, f"1d-{n}")
+    if len(dims) == 2:
+        x = _signal(rng, dims, 40, kmax=min(127, (min(dims) - 1) // 2))
+        return _finish(dims, x, 0.01, rng, _lattice_balls(dims, 16, 3), 0.0, 0.05,
+                       "x".join(map(str, dims)))
+    x = _signal(rng, dims, 200, kmax=min(127, (min(dims) - 1) // 2))
+    return _finish(dims, x, 0.01, rng, _lattice_balls(dims, 8, 2), 0.01, 0.05,
+                   "x".join(map(str, dims)))
 
 
 def config4_stream(nreq=10_000, rate=50.0):
