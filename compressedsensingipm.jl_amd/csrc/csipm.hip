@@ -255,12 +255,25 @@ static int upload_twiddles(int L, void** out) {
 // the registers (lines*L <= NT*E) and 64 KB of LDS allow, fewer when that
 // would leave the launch with under min_grid workgroups (1024 for one
 // instance; a batch shares the target among its instances)
+// lines per workgroup below which a row of the tile is under one 128-byte
+// cache line (the reads of a line pass then fetch twice what they use)
+static int min_tile_lg(size_t cs) {
+  int lg = 0;
+  while ((size_t)(1 << lg) * cs < 128) ++lg;
+  return lg;
+}
+
 static LineCfg line_cfg(int H1, int L, int64_t others, size_t cs, int min_grid = 1024) {
   LineCfg c{};
   int lg = 0;
   while ((2 << lg) * L <= kNT * kE && (size_t)(2 << lg) * L * cs <= kLdsSmall && (1 << lg) < H1)
     ++lg;
-  while (lg > 0 && (int64_t)((H1 + (1 << lg) - 1) >> lg) * others < min_grid) --lg;
+  // below one cache line per row only while the launch would not even give a
+  // quarter of the workgroup target (small single instances: latency first)
+  const int lg_min = std::min(lg, min_tile_lg(cs));
+  auto grid_of = [&](int l) { return (int64_t)((H1 + (1 << l) - 1) >> l) * others; };
+  while (lg > 0 && grid_of(lg) < min_grid && !(lg <= lg_min && 4 * grid_of(lg) >= min_grid))
+    --lg;
   c.lg = lg;
   c.ntile1 = (H1 + (1 << lg) - 1) >> lg;
   c.grid = (int)(c.ntile1 * others);
@@ -274,7 +287,9 @@ static LineCfg col_cfg(int A, int B, size_t cs, int min_grid = 1024) {
   int lg = 0;
   while ((2 << lg) * B <= kNT * kE && (size_t)(2 << lg) * B * cs <= kLdsSmall && (2 << lg) <= A)
     ++lg;
-  while (lg > 0 && ((A + (1 << lg) - 1) >> lg) < min_grid) --lg;
+  const int lg_min = std::min(lg, min_tile_lg(cs));
+  auto grid_of = [&](int l) { return (A + (1 << l) - 1) >> l; };
+  while (lg > 0 && grid_of(lg) < min_grid && !(lg <= lg_min && 4 * grid_of(lg) >= min_grid)) --lg;
   c.lg = lg;
   c.ntile1 = (A + (1 << lg) - 1) >> lg;
   c.grid = c.ntile1;
