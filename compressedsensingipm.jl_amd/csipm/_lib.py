@@ -51,6 +51,7 @@ SIGNATURES = {
     "csipm_batch_apply_timed": (_i, [_vp, _i64, ctypes.POINTER(_d), ctypes.POINTER(_d), _i,
                                      ctypes.POINTER(_i)]),
     "csipm_batch_launch_name": (ctypes.c_char_p, [_vp, _i]),
+    "csipm_batch_stats": (_i, [_vp, ctypes.POINTER(_d)]),
     "csipm_batch_solve": (_i, [_vp, _i, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
                                ctypes.POINTER(_d), _d, _i64, _d, _i64, ctypes.POINTER(_vp),
                                ctypes.POINTER(_d)]),

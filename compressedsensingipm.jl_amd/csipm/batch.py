@@ -123,6 +123,14 @@ class BatchSolver:
     def live(self):
         return len(self._live)
 
+    def stats(self):
+        """Host-side accounting of the engine (seconds / counts)."""
+        out = (ctypes.c_double * 8)()
+        _lib.call("csipm_batch_stats", self._b, out)
+        keys = ("step_s", "tables_s", "capture_s", "init_s", "cg_s", "table_builds",
+                "captures", "steps")
+        return {k: out[i] for i, k in enumerate(keys)}
+
     def apply_timed(self, nrep=20):
         """Per-launch HIP-event times of the batch's CG operator apply:
         list of (name, ms, algorithmic bytes)."""

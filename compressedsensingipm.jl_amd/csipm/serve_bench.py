@@ -107,12 +107,14 @@ def main():
                 live_bytes.pop(k, None)
                 ops.pop(k, None)
         span = time.perf_counter() - t_start
+        engine_stats = bs.stats()
         bs.close()
         lat = [res_b[k].t_done - arrivals[k] for k in res_b]
         out["batched_mixed"] = {"solves_per_s": round(len(res_b) / span, 2),
                                 "makespan_s": round(span, 2), "p50_ms": pct(lat, 50),
                                 "p99_ms": pct(lat, 99),
-                                "all_solved": all(r.status == "solved" for r in res_b.values())}
+                                "all_solved": all(r.status == "solved" for r in res_b.values()),
+                                "engine": {k: round(v, 3) for k, v in engine_stats.items()}}
         print(f"# batched: {span:.1f} s", file=sys.stderr, flush=True)
 
     res_p = {}

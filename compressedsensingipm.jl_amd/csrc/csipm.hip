@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <initializer_list>
+#include <memory>
 #include <mutex>
 #include <unordered_map>
 #include <string>
@@ -178,8 +179,20 @@ static bool plan_axis(int L, FftPlan& p, bool& bs) {
   return factorize(M, p);
 }
 
+// exp(-2 pi i k / M), k < M/2, in extended precision rounded once to FP64
+static std::vector<std::complex<double>> host_roots(size_t M) {
+  std::vector<std::complex<double>> r(M / 2);
+  const long double pi = acosl(-1.0L);
+  for (size_t k = 0; k < M / 2; ++k) {
+    const long double ang = -2.0L * pi * (long double)k / (long double)M;
+    r[k] = std::complex<double>((double)cosl(ang), (double)sinl(ang));
+  }
+  return r;
+}
+
 // in-place forward DFT (exp(-2 pi i jk/M)) of a power-of-two length, host FP64
-static void host_fft_pow2(std::vector<std::complex<double>>& a) {
+static void host_fft_pow2(std::vector<std::complex<double>>& a,
+                          const std::vector<std::complex<double>>& roots) {
   const size_t M = a.size();
   for (size_t i = 1, j = 0; i < M; ++i) {
     size_t bit = M >> 1;
@@ -187,67 +200,106 @@ static void host_fft_pow2(std::vector<std::complex<double>>& a) {
     j ^= bit;
     if (i < j) std::swap(a[i], a[j]);
   }
-  const long double pi = acosl(-1.0L);
   for (size_t len = 2; len <= M; len <<= 1) {
-    const size_t half = len / 2;
-    for (size_t k = 0; k < half; ++k) {
-      const long double ang = -2.0L * pi * (long double)k / (long double)len;
-      const std::complex<double> w((double)cosl(ang), (double)sinl(ang));
-      for (size_t i = 0; i < M; i += len) {
-        const std::complex<double> u = a[i + k], v = a[i + k + half] * w;
+    const size_t half = len / 2, stride = M / len;
+    for (size_t i = 0; i < M; i += len)
+      for (size_t k = 0; k < half; ++k) {
+        const std::complex<double> u = a[i + k], v = a[i + k + half] * roots[k * stride];
         a[i + k] = u + v;
         a[i + k + half] = u - v;
       }
-    }
   }
 }
 
 // Bluestein tables of line length L, convolution length M: the chirp
 // w_n = exp(-i pi (n^2 mod 2L) / L) and the kernel spectra FFT_M(b)/M of the
 // forward (b = conj w) and inverse (b = w) convolution kernels, wrapped so
-// that b_{M-m} = b_m
-template <typename T>
-static int upload_bluestein(int L, int M, void** chirp, void** bfw, void** binv) {
+// that b_{M-m} = b_m.  Computed once in FP64 per (L, M) (the last pair is
+// kept, so the two precisions of a mixed handle share one computation).
+struct BluesteinHost {
+  int L = 0, M = 0;
+  std::vector<std::complex<double>> w, bf, bi;
+};
+
+static std::shared_ptr<const BluesteinHost> bluestein_host(int L, int M) {
+  static std::mutex mu;
+  static std::shared_ptr<const BluesteinHost> last;
+  {
+    std::lock_guard<std::mutex> lock(mu);
+    if (last && last->L == L && last->M == M) return last;
+  }
+  auto tp = std::make_shared<BluesteinHost>();
+  BluesteinHost& t = *tp;
+  t.L = L;
+  t.M = M;
   const long double pi = acosl(-1.0L);
-  std::vector<std::complex<double>> w(L);
+  t.w.resize((size_t)L);
   for (int n = 0; n < L; ++n) {
     const long long q = ((long long)n * n) % (2LL * L);
     const long double ang = -pi * (long double)q / (long double)L;
-    w[n] = std::complex<double>((double)cosl(ang), (double)sinl(ang));
+    t.w[(size_t)n] = std::complex<double>((double)cosl(ang), (double)sinl(ang));
   }
-  std::vector<cpx<T>> hc(L);
-  for (int n = 0; n < L; ++n) hc[n] = {(T)w[n].real(), (T)w[n].imag()};
-  HIPCHK(hipMalloc(chirp, sizeof(cpx<T>) * (size_t)L));
-  HIPCHK(hipMemcpy(*chirp, hc.data(), sizeof(cpx<T>) * (size_t)L, hipMemcpyHostToDevice));
+  const std::vector<std::complex<double>> roots = host_roots((size_t)M);
   for (int dir = 0; dir < 2; ++dir) {
     std::vector<std::complex<double>> b((size_t)M, 0.0);
     for (int m = 0; m < L; ++m) {
-      const std::complex<double> v = dir == 0 ? std::conj(w[m]) : w[m];
+      const std::complex<double> v = dir == 0 ? std::conj(t.w[(size_t)m]) : t.w[(size_t)m];
       b[(size_t)m] = v;
       if (m > 0) b[(size_t)(M - m)] = v;
     }
-    host_fft_pow2(b);
-    std::vector<cpx<T>> hb((size_t)M);
-    for (int m = 0; m < M; ++m)
-      hb[(size_t)m] = {(T)(b[(size_t)m].real() / M), (T)(b[(size_t)m].imag() / M)};
-    void** out = dir == 0 ? bfw : binv;
-    HIPCHK(hipMalloc(out, sizeof(cpx<T>) * (size_t)M));
-    HIPCHK(hipMemcpy(*out, hb.data(), sizeof(cpx<T>) * (size_t)M, hipMemcpyHostToDevice));
+    host_fft_pow2(b, roots);
+    for (auto& x : b) x /= (double)M;
+    (dir == 0 ? t.bf : t.bi).swap(b);
   }
+  std::lock_guard<std::mutex> lock(mu);
+  last = tp;
+  return last;
+}
+
+template <typename T>
+static int upload_cpx(const std::vector<std::complex<double>>& v, void** out) {
+  std::vector<cpx<T>> h(v.size());
+  for (size_t i = 0; i < v.size(); ++i) h[i] = {(T)v[i].real(), (T)v[i].imag()};
+  HIPCHK(hipMalloc(out, sizeof(cpx<T>) * h.size()));
+  HIPCHK(hipMemcpy(*out, h.data(), sizeof(cpx<T>) * h.size(), hipMemcpyHostToDevice));
   return CSIPM_OK;
 }
 
 template <typename T>
+static int upload_bluestein(int L, int M, void** chirp, void** bfw, void** binv) {
+  const std::shared_ptr<const BluesteinHost> t = bluestein_host(L, M);
+  CHK(upload_cpx<T>(t->w, chirp));
+  CHK(upload_cpx<T>(t->bf, bfw));
+  CHK(upload_cpx<T>(t->bi, binv));
+  return CSIPM_OK;
+}
+
+// exp(-2 pi i t / L), t < L, in the op precision.  Tables are shared by all
+// handles on a device and kept for the life of the process (one per length
+// and precision; a serving process reuses them across requests).
+template <typename T>
 static int upload_twiddles(int L, void** out) {
-  std::vector<cpx<T>> h(L);
+  static std::mutex mu;
+  static std::unordered_map<int64_t, void*> cache;
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  const int64_t key = ((int64_t)dev << 40) | ((int64_t)sizeof(T) << 32) | (int64_t)L;
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = cache.find(key);
+  if (it != cache.end()) {
+    *out = it->second;
+    return CSIPM_OK;
+  }
+  std::vector<cpx<T>> h((size_t)L);
   const long double two_pi = 2.0L * acosl(-1.0L);
   for (int t = 0; t < L; ++t) {
     const long double a = two_pi * (long double)t / (long double)L;
-    h[t].x = (T)cosl(a);
-    h[t].y = (T)(-sinl(a));
+    h[(size_t)t].x = (T)cosl(a);
+    h[(size_t)t].y = (T)(-sinl(a));
   }
   HIPCHK(hipMalloc(out, sizeof(cpx<T>) * (size_t)L));
   HIPCHK(hipMemcpy(*out, h.data(), sizeof(cpx<T>) * (size_t)L, hipMemcpyHostToDevice));
+  cache[key] = *out;
   return CSIPM_OK;
 }
 
@@ -839,13 +891,11 @@ const char* csipm_last_error(void) { return g_err.c_str(); }
 
 int csipm_op_destroy(csipm_op_t h) {
   if (!h) return CSIPM_OK;
+  // twiddle tables (tw, twA, twB) are shared across handles and stay
   for (int i = 0; i < 3; ++i)
     for (PrecState* ps : {&h->p64, &h->p32})
-      for (void* b : {ps->tw[i], ps->chirp[i], ps->bfw[i], ps->binv[i]})
+      for (void* b : {ps->chirp[i], ps->bfw[i], ps->binv[i]})
         if (b) (void)hipFree(b);
-  for (PrecState* ps : {&h->p64, &h->p32})
-    for (void* b : {ps->twA, ps->twB})
-      if (b) (void)hipFree(b);
   delete h->solo;
   void* bufs[] = {h->miss, h->valid, h->buf, h->part, h->scratch};
   for (void* b : bufs)
@@ -1483,6 +1533,12 @@ int csipm_batch_apply_timed(csipm_batch_t b, int64_t nrep, double* pass_ms,
 const char* csipm_batch_launch_name(csipm_batch_t b, int k) {
   if (!b || k < 0 || k >= (int)b->names.size()) return "";
   return b->names[(size_t)k].c_str();
+}
+
+int csipm_batch_stats(csipm_batch_t b, double* out) {
+  if (!b || !out) return fail(CSIPM_ERR_ARG, "null argument");
+  for (int i = 0; i < Engine::ST_N; ++i) out[i] = b->E->stats[i];
+  return CSIPM_OK;
 }
 
 }  // extern "C"

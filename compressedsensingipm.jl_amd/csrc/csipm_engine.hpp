@@ -209,6 +209,10 @@ struct Engine {
   std::vector<int64_t> table_sig;  // (id, handle, offset) of the set the tables were built for
   int table_nb = 0;
   bool table_ipm = false;
+  // host-side seconds and counts, for csipm_batch_stats
+  enum { ST_STEP = 0, ST_TABLES, ST_CAPTURE, ST_INIT, ST_CG, ST_NTABLES, ST_NCAPTURE, ST_NSTEP,
+         ST_N };
+  double stats[ST_N] = {};
   int64_t cap = 0;          // pool slots
   int maxinst = 0;
   struct Inst {
@@ -273,6 +277,15 @@ struct Engine {
       if (p) (void)hipFree(p);
     if (flags_h) (void)hipHostFree(flags_h);
     if (cg_h) (void)hipHostFree(cg_h);
+  }
+};
+
+struct StatTimer {
+  double* slot;
+  std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+  explicit StatTimer(double* s) : slot(s) {}
+  ~StatTimer() {
+    *slot += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   }
 };
 
@@ -607,6 +620,8 @@ int engine_tables(Engine* E, int nb, bool ipm) {
     E->tables_dirty = false;
   }
   if (!E->tables_dirty) return CSIPM_OK;
+  StatTimer tm(&E->stats[Engine::ST_TABLES]);
+  E->stats[Engine::ST_NTABLES] += 1;
   E->release_tables();
   E->table_sig.clear();
   CHK(upload_vseg(E, E->active, &E->seg_mem, &E->vseg, &E->ids, &E->vgrid));
@@ -702,6 +717,7 @@ int engine_init_fresh(Engine* E) {
   for (int id : E->active)
     if (E->inst[(size_t)id].fresh) fresh.push_back(id);
   if (fresh.empty()) return CSIPM_OK;
+  StatTimer tm(&E->stats[Engine::ST_INIT]);
   Chain mtz;
   int* mem = nullptr;
   BSeg seg{};
@@ -772,6 +788,8 @@ constexpr int kCgChunk = 4;   // CG iterations between host checks (one graph)
 // capture the two CG chunks for the current tables
 template <typename TC>
 int capture_cg_graphs(Engine* E, int nb, int reliable) {
+  StatTimer tm(&E->stats[Engine::ST_CAPTURE]);
+  E->stats[Engine::ST_NCAPTURE] += 1;
   E->release_graphs();
   for (int g = 0; g < (reliable > 0 ? 2 : 1); ++g) {
     hipGraph_t graph = nullptr;
@@ -807,6 +825,7 @@ int capture_cg_graphs(Engine* E, int nb, int reliable) {
 // ---------------------------------------------------------------------------
 template <typename TC>
 int engine_cg(Engine* E, int nb, double atol, double rtol, int reliable) {
+  StatTimer tm(&E->stats[Engine::ST_CG]);
   const int na = (int)E->active.size();
   TC *b = E->C2<TC>(C2_RHS), *x = E->C2<TC>(C2_D), *r = E->C2<TC>(C2_R);
   TC *z = E->C2<TC>(C2_Z), *p = E->C2<TC>(C2_P);
@@ -887,6 +906,8 @@ int engine_newton(Engine* E, double rtol, int reliable, bool refine) {
 // ---------------------------------------------------------------------------
 template <typename TC>
 int engine_step(Engine* E, double tol, int max_iter, double cg_rtol, int* running) {
+  StatTimer tm(&E->stats[Engine::ST_STEP]);
+  E->stats[Engine::ST_NSTEP] += 1;
   const int nb = E->mixed ? 1 : 2;
   CHK(engine_init_fresh(E));
   CHK(engine_tables<TC>(E, nb, true));

@@ -222,6 +222,11 @@ int csipm_batch_solve(csipm_batch_t b, int n, const csipm_op_t* ops,
 int csipm_batch_apply_timed(csipm_batch_t b, int64_t nrep, double* pass_ms,
                             double* pass_bytes, int cap, int* nlaunch);
 const char* csipm_batch_launch_name(csipm_batch_t b, int k);
+/* Host-side accounting of the engine since creation, out[8] = seconds in
+ * csipm_batch_step, of which in launch-table builds, graph captures,
+ * initialisation of admitted instances and CG solves; then the numbers of
+ * table builds, graph captures and steps.                                */
+int csipm_batch_stats(csipm_batch_t b, double* out);
 
 #ifdef __cplusplus
 }
