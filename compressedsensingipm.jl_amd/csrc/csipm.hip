@@ -107,6 +107,7 @@ struct csipm_op {
   bool big = false, obs = false;
   int bA = 0, bB = 0, bF = 0;
   FftPlan planA{}, planB{};
+  FftPlan* dplan = nullptr;      // device copies: plan[0..2], planA, planB
   PrecState p64, p32;
   uint32_t* miss = nullptr;
   int64_t nmiss = 0;
@@ -412,7 +413,7 @@ template <typename T>
 PassArgs<T> base_args(csipm_op* h, int axis) {
   PassArgs<T> a{};
   a.g = h->g;
-  a.plan = h->plan[axis];
+  a.plan = h->dplan + axis;
   a.tw = (const cpx<T>*)h->ps<T>().tw[axis];
   a.H1 = h->H1;
   a.buf = (cpx<T>*)h->buf;
@@ -510,13 +511,13 @@ PassArgs<T> big_args(csipm_op* h, bool col, const void* src, const T* real_in, T
   a.bs = 0;
   if (col) {
     const LineCfg& c = cc ? *cc : ps.col;
-    a.plan = h->planB;
+    a.plan = h->dplan + 4;
     a.tw = (const cpx<T>*)ps.twB;
     a.lg = c.lg;
     a.ps = 1 << c.lg;
   } else {
     const Dim1Cfg& c = rc ? *rc : ps.row;
-    a.plan = h->planA;
+    a.plan = h->dplan + 3;
     a.tw = (const cpx<T>*)ps.twA;
     a.lg = c.lg;
     a.ps = c.ps;
@@ -897,7 +898,7 @@ int csipm_op_destroy(csipm_op_t h) {
       for (void* b : {ps->chirp[i], ps->bfw[i], ps->binv[i]})
         if (b) (void)hipFree(b);
   delete h->solo;
-  void* bufs[] = {h->miss, h->valid, h->buf, h->part, h->scratch};
+  void* bufs[] = {h->miss, h->valid, h->buf, h->part, h->scratch, h->dplan};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   delete h;
@@ -1015,6 +1016,11 @@ static int create_impl(csipm_op* h, int dim, const int64_t sizes[3], int rdft, i
         CHK(upload_bluestein<float>(h->n[i], F, &h->p32.chirp[i], &h->p32.bfw[i],
                                     &h->p32.binv[i]));
     }
+  }
+  {
+    const FftPlan plans[5] = {h->plan[0], h->plan[1], h->plan[2], h->planA, h->planB};
+    HIPCHK(hipMalloc(&h->dplan, sizeof(plans)));
+    HIPCHK(hipMemcpy(h->dplan, plans, sizeof(plans), hipMemcpyHostToDevice));
   }
   if (h->has64()) CHK(set_shapes(h, h->p64, sizeof(cpx<double>)));
   if (h->has32()) CHK(set_shapes(h, h->p32, sizeof(cpx<float>)));

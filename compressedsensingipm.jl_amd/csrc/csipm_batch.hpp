@@ -35,9 +35,10 @@ struct BFlags {
 };
 
 #define BSEG_BEGIN                                                  \
-  const int inst = seg.inst[blockIdx.x];                            \
-  const int lb = blockIdx.x - seg.first[inst];                      \
-  const int nbk = seg.first[inst + 1] - seg.first[inst];            \
+  const int inst = uniform(seg.inst[blockIdx.x]);                   \
+  const int f0_ = uniform(seg.first[inst]);                         \
+  const int lb = (int)blockIdx.x - f0_;                             \
+  const int nbk = uniform(seg.first[inst + 1]) - f0_;               \
   const BInst& I = info[inst];                                      \
   if (flags && flags[inst].done) return;
 #define BFOR(i)                                                                     \

@@ -344,6 +344,47 @@ __device__ __forceinline__ void dft_odd(cpx<T>* v) {
   v[0] = y0;
 }
 
+// the same, writing X[k] to dst[k * stride] as it is formed: a_j and b_j
+// overwrite the inputs in place and no output array is held, which keeps the
+// large odd radices (11, 13, 15, 17) within the register budget of the small
+// ones
+template <typename T, int R, bool FWD>
+__device__ __forceinline__ void dft_odd_store(cpx<T>* v, cpx<T>* __restrict__ dst, int stride) {
+  constexpr int H = (R - 1) / 2;
+  const cpx<T> x0 = v[0];
+  cpx<T> y0 = x0;
+#pragma unroll
+  for (int j = 1; j <= H; ++j) {
+    const cpx<T> a = cadd(v[j], v[R - j]);
+    const cpx<T> b = csub(v[j], v[R - j]);
+    v[j] = a;
+    v[R - j] = b;
+    y0 = cadd(y0, a);
+  }
+  dst[0] = y0;
+#pragma unroll
+  for (int k = 1; k <= H; ++k) {
+    T re = x0.x, im = x0.y, sre = 0, sim = 0;
+#pragma unroll
+    for (int j = 1; j <= H; ++j) {
+      const int m = (j * k) % R;
+      const T c = (T)OddRadix<R>::c[m];
+      const T sn = (T)OddRadix<R>::s[m];
+      re += v[j].x * c;
+      im += v[j].y * c;
+      sre += v[R - j].y * sn;
+      sim += v[R - j].x * sn;
+    }
+    if (FWD) {
+      dst[k * stride] = {re + sre, im - sim};
+      dst[(R - k) * stride] = {re - sre, im + sim};
+    } else {
+      dst[k * stride] = {re - sre, im + sim};
+      dst[(R - k) * stride] = {re + sre, im - sim};
+    }
+  }
+}
+
 template <typename T, int R, bool FWD>
 __device__ __forceinline__ void dft(cpx<T>* v) {
   if constexpr (R == 2) dft2<T, FWD>(v);
@@ -375,6 +416,9 @@ struct FftPlan {
   uint32_t nsm[kMaxStages];  // ceil(2^32 / ns), ns >= 2: j / ns = umulhi(j, nsm)
 };
 
+// a wave-uniform value into a scalar register (the value must be uniform)
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
 // exact for a, d <= 2^16 (host checks the plan limits)
 __device__ __forceinline__ int fdiv(int a, uint32_t m) { return (int)__umulhi((uint32_t)a, m); }
 
@@ -382,15 +426,22 @@ template <typename T, int R, int E, int NT, bool FWD>
 __device__ __forceinline__ void stockham_stage(cpx<T>* __restrict__ lds, const FftPlan& p, int s,
                                                int lg, int ps, const cpx<T>* __restrict__ tw) {
   constexpr int NQ = (E + R - 1) / R;
-  const int L = p.L;
+  const int L = uniform(p.L);
   const int nb = L / R;
   const int nl = 1 << lg;
   const int total = nb << lg;
-  const int Ns = p.ns[s];
+  const int Ns = uniform(p.ns[s]);
+  const int tws = uniform(p.tws[s]);
+  const uint32_t nsm = (uint32_t)uniform((int)p.nsm[s]);
+  // an opaque copy of the lane index: keeps the compiler from hoisting every
+  // stage's per-lane address arithmetic out of the caller's stage loop (the
+  // hoisted values of all radix cases would stay live across the whole FFT)
+  int tid = (int)threadIdx.x;
+  asm volatile("" : "+v"(tid));
   cpx<T> v[NQ][R];
 #pragma unroll
   for (int q = 0; q < NQ; ++q) {
-    const int b = threadIdx.x + q * NT;
+    const int b = tid + q * NT;
     if (b < total) {
       const int j = b >> lg;
       const int line = b & (nl - 1);
@@ -402,14 +453,14 @@ __device__ __forceinline__ void stockham_stage(cpx<T>* __restrict__ lds, const F
   __syncthreads();
 #pragma unroll
   for (int q = 0; q < NQ; ++q) {
-    const int b = threadIdx.x + q * NT;
+    const int b = tid + q * NT;
     if (b < total) {
       const int j = b >> lg;
       const int line = b & (nl - 1);
       int k = 0;
       if (Ns > 1) {
-        k = j - fdiv(j, p.nsm[s]) * Ns;
-        const int step = k * p.tws[s];
+        k = j - fdiv(j, nsm) * Ns;
+        const int step = k * tws;
 #pragma unroll
         for (int r = 1; r < R; ++r) {
           cpx<T> w = tw[r * step];
@@ -417,20 +468,25 @@ __device__ __forceinline__ void stockham_stage(cpx<T>* __restrict__ lds, const F
           v[q][r] = cmul(v[q][r], w);
         }
       }
-      dft<T, R, FWD>(v[q]);
       cpx<T>* dst = lds + ((j - k) * R + k) * ps + line;
+      if constexpr (R == 2 || R == 4 || R == 8 || R == 16) {
+        dft<T, R, FWD>(v[q]);
 #pragma unroll
-      for (int r = 0; r < R; ++r) dst[r * Ns * ps] = v[q][r];
+        for (int r = 0; r < R; ++r) dst[r * Ns * ps] = v[q][r];
+      } else {
+        dft_odd_store<T, R, FWD>(v[q], dst, Ns * ps);
+      }
     }
   }
   __syncthreads();
 }
 
 template <typename T, int E, int NT, bool FWD>
-__device__ void lds_fft(cpx<T>* lds, const FftPlan& plan, int lg, int ps,
+__device__ __forceinline__ void lds_fft(cpx<T>* lds, const FftPlan& plan, int lg, int ps,
                         const cpx<T>* __restrict__ tw) {
-  for (int s = 0; s < plan.nstages; ++s) {
-    switch (plan.radix[s]) {
+  const int nst = uniform(plan.nstages);
+  for (int s = 0; s < nst; ++s) {
+    switch (uniform(plan.radix[s])) {
       case 2: stockham_stage<T, 2, E, NT, FWD>(lds, plan, s, lg, ps, tw); break;
       case 3: stockham_stage<T, 3, E, NT, FWD>(lds, plan, s, lg, ps, tw); break;
       case 4: stockham_stage<T, 4, E, NT, FWD>(lds, plan, s, lg, ps, tw); break;

@@ -294,7 +294,7 @@ enum { MODE_MTM = 0, MODE_C2R = 1, MODE_R2C = 2 };
 template <typename T>
 struct PassArgs {
   MapGeom g;
-  FftPlan plan;
+  const FftPlan* plan;  // device-resident (read with uniform loads)
   const cpx<T>* tw;
   int axis;      // k_lines: 1 or 2
   int H1;
@@ -335,12 +335,12 @@ struct PassArgs {
 // inverse conjugates w), the convolution by FFT_M, pointwise product with the
 // precomputed kernel spectrum, inverse FFT_M.  LDS must hold M rows.
 template <typename T, int E, int NT, bool FWD>
-__device__ void line_dft(cpx<T>* lds, const PassArgs<T>& a, int lg, int ps) {
+__device__ __forceinline__ void line_dft(cpx<T>* lds, const PassArgs<T>& a, int lg, int ps) {
   if (!a.bs) {
-    lds_fft<T, E, NT, FWD>(lds, a.plan, lg, ps, a.tw);
+    lds_fft<T, E, NT, FWD>(lds, *a.plan, lg, ps, a.tw);
     return;
   }
-  const int L = a.len, M = a.plan.L, nl = 1 << lg;
+  const int L = a.len, M = uniform(a.plan->L), nl = 1 << lg;
   for (int idx = threadIdx.x; idx < (M << lg); idx += NT) {
     const int line = idx & (nl - 1), e = idx >> lg;
     cpx<T>* q = lds + e * ps + line;
@@ -353,7 +353,7 @@ __device__ void line_dft(cpx<T>* lds, const PassArgs<T>& a, int lg, int ps) {
     }
   }
   __syncthreads();
-  lds_fft<T, E, NT, true>(lds, a.plan, lg, ps, a.tw);
+  lds_fft<T, E, NT, true>(lds, *a.plan, lg, ps, a.tw);
   const cpx<T>* B = FWD ? a.bfw : a.binv;
   for (int idx = threadIdx.x; idx < (M << lg); idx += NT) {
     const int line = idx & (nl - 1), e = idx >> lg;
@@ -361,7 +361,7 @@ __device__ void line_dft(cpx<T>* lds, const PassArgs<T>& a, int lg, int ps) {
     *q = cmul(*q, B[e]);
   }
   __syncthreads();
-  lds_fft<T, E, NT, false>(lds, a.plan, lg, ps, a.tw);
+  lds_fft<T, E, NT, false>(lds, *a.plan, lg, ps, a.tw);
   for (int idx = threadIdx.x; idx < (L << lg); idx += NT) {
     const int line = idx & (nl - 1), e = idx >> lg;
     cpx<T>* q = lds + e * ps + line;
@@ -451,8 +451,9 @@ struct BSeg {
 
 template <typename T, int NT, int E, int D, int SRC, int DST, bool FWD>
 __global__ __launch_bounds__(NT) void kb_lines(const PassArgs<T>* __restrict__ tab, BSeg seg) {
-  const int i = seg.inst[blockIdx.x];
-  lines_body<T, NT, E, D, SRC, DST, FWD>(tab[i], blockIdx.x - seg.first[i]);
+  const int i = uniform(seg.inst[blockIdx.x]);
+  const PassArgs<T> a = tab[i];  // loaded before any store: stays in scalar registers
+  lines_body<T, NT, E, D, SRC, DST, FWD>(a, (int)blockIdx.x - uniform(seg.first[i]));
 }
 
 __device__ __forceinline__ bool miss_bit(const uint32_t* m, int64_t s) {
@@ -553,8 +554,9 @@ __global__ __launch_bounds__(NT) void k_dim1(PassArgs<T> a) {
 
 template <typename T, int NT, int E, int D, int SRC, int DST, int MODE>
 __global__ __launch_bounds__(NT) void kb_dim1(const PassArgs<T>* __restrict__ tab, BSeg seg) {
-  const int i = seg.inst[blockIdx.x];
-  dim1_body<T, NT, E, D, SRC, DST, MODE>(tab[i], blockIdx.x - seg.first[i]);
+  const int i = uniform(seg.inst[blockIdx.x]);
+  const PassArgs<T> a = tab[i];  // loaded before any store: stays in scalar registers
+  dim1_body<T, NT, E, D, SRC, DST, MODE>(a, (int)blockIdx.x - uniform(seg.first[i]));
 }
 
 // ===========================================================================
@@ -628,8 +630,8 @@ __device__ __forceinline__ void col_body(const PassArgs<T>& a, int tile) {
   }
   __syncthreads();
   const bool fwd1 = MODE == C_FIRST ? bs : (MODE == C_REAL_IN ? true : (MODE == C_LAST ? !bs : false));
-  if (fwd1) lds_fft<T, E, NT, true>(lds, a.plan, lg, nl, a.tw);
-  else lds_fft<T, E, NT, false>(lds, a.plan, lg, nl, a.tw);
+  if (fwd1) lds_fft<T, E, NT, true>(lds, *a.plan, lg, nl, a.tw);
+  else lds_fft<T, E, NT, false>(lds, *a.plan, lg, nl, a.tw);
   if (MODE == C_MID || MODE == C_MID_C2R) {
     // conv -> x_n = conj(w_n) conv_n, masked real part; then (MTM) x w for the forward
     for (int idx = threadIdx.x; idx < (B << lg); idx += NT) {
@@ -647,7 +649,7 @@ __device__ __forceinline__ void col_body(const PassArgs<T>& a, int tile) {
     }
     if (MODE == C_MID_C2R) return;
     __syncthreads();
-    lds_fft<T, E, NT, true>(lds, a.plan, lg, nl, a.tw);
+    lds_fft<T, E, NT, true>(lds, *a.plan, lg, nl, a.tw);
   }
   double acc = 0.0;
   for (int idx = threadIdx.x; idx < (B << lg); idx += NT) {
@@ -692,8 +694,8 @@ __device__ __forceinline__ void row_body(const PassArgs<T>& a, int tile) {
   }
   __syncthreads();
   const bool fwd1 = MODE == R_R2C || MODE == R_CONV_INV || MODE == R_CONV_FWD;
-  if (fwd1) lds_fft<T, E, NT, true>(lds, a.plan, lg, ps, a.tw);
-  else lds_fft<T, E, NT, false>(lds, a.plan, lg, ps, a.tw);
+  if (fwd1) lds_fft<T, E, NT, true>(lds, *a.plan, lg, ps, a.tw);
+  else lds_fft<T, E, NT, false>(lds, *a.plan, lg, ps, a.tw);
   if (MODE == R_MTM || MODE == R_C2R) {
     for (int idx = threadIdx.x; idx < tot; idx += NT) {
       const int line = udiv(idx, A, a.am);
@@ -709,7 +711,7 @@ __device__ __forceinline__ void row_body(const PassArgs<T>& a, int tile) {
     }
     if (MODE == R_C2R) return;
     __syncthreads();
-    lds_fft<T, E, NT, true>(lds, a.plan, lg, ps, a.tw);
+    lds_fft<T, E, NT, true>(lds, *a.plan, lg, ps, a.tw);
   } else if (MODE == R_CONV_INV || MODE == R_CONV_FWD) {
     const cpx<T>* Bh = MODE == R_CONV_FWD ? a.bfw : a.binv;
     for (int idx = threadIdx.x; idx < tot; idx += NT) {
@@ -719,7 +721,7 @@ __device__ __forceinline__ void row_body(const PassArgs<T>& a, int tile) {
       if (line < nv) *q = cmul(*q, Bh[r0 + line + (int64_t)B * c]);
     }
     __syncthreads();
-    lds_fft<T, E, NT, false>(lds, a.plan, lg, ps, a.tw);
+    lds_fft<T, E, NT, false>(lds, *a.plan, lg, ps, a.tw);
   }
   const bool inv_tw = MODE == R_CONV_INV || MODE == R_CONV_FWD;
   for (int idx = threadIdx.x; idx < tot; idx += NT) {
@@ -737,8 +739,9 @@ __global__ __launch_bounds__(NT) void k_col(PassArgs<T> a) {
 }
 template <typename T, int NT, int E, int SRC, int DST, int MODE>
 __global__ __launch_bounds__(NT) void kb_col(const PassArgs<T>* __restrict__ tab, BSeg seg) {
-  const int i = seg.inst[blockIdx.x];
-  col_body<T, NT, E, SRC, DST, MODE>(tab[i], blockIdx.x - seg.first[i]);
+  const int i = uniform(seg.inst[blockIdx.x]);
+  const PassArgs<T> a = tab[i];  // loaded before any store: stays in scalar registers
+  col_body<T, NT, E, SRC, DST, MODE>(a, (int)blockIdx.x - uniform(seg.first[i]));
 }
 template <typename T, int NT, int E, int MODE>
 __global__ __launch_bounds__(NT) void k_row(PassArgs<T> a) {
@@ -746,8 +749,9 @@ __global__ __launch_bounds__(NT) void k_row(PassArgs<T> a) {
 }
 template <typename T, int NT, int E, int MODE>
 __global__ __launch_bounds__(NT) void kb_row(const PassArgs<T>* __restrict__ tab, BSeg seg) {
-  const int i = seg.inst[blockIdx.x];
-  row_body<T, NT, E, MODE>(tab[i], blockIdx.x - seg.first[i]);
+  const int i = uniform(seg.inst[blockIdx.x]);
+  const PassArgs<T> a = tab[i];  // loaded before any store: stays in scalar registers
+  row_body<T, NT, E, MODE>(a, (int)blockIdx.x - uniform(seg.first[i]));
 }
 
 // beta <-> u (exact: a signed permutation), one thread per half-spectrum element.
