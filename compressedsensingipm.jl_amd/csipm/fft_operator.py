@@ -58,14 +58,14 @@ class FFTOperator:
 
     def __init__(self, nbeta, DFTdim, DFTsize, index_missing, rdft=True, dtype=torch.float64,
                  device="cuda", mixed=False):
-        if not rdft:
-            raise NotImplementedError("only the rdft=true (R2C/C2R) path is implemented")
         self.DFTdim = int(DFTdim)
         self.DFTsize = tuple(int(s) for s in DFTsize)
         self.N = int(np.prod(self.DFTsize))
         if int(nbeta) != self.N:
             raise ValueError("nbeta must equal prod(DFTsize)")
-        self.rdft = True
+        # rdft=False (the reference's default, fft_utils.jl:46-48): the same
+        # operator; the maps default to the full spectrum
+        self.rdft = bool(rdft)
         if mixed:
             dtype = torch.float32     # the operator entry points run in FP32
         if dtype not in (torch.float64, torch.float32):
@@ -80,7 +80,7 @@ class FFTOperator:
         mp = miss.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)) if miss.size else None
         h = ctypes.c_void_p()
         with torch.cuda.device(self.device):
-            _lib.call("csipm_op_create", ctypes.byref(h), self.DFTdim, sizes, 1, prec, mp,
+            _lib.call("csipm_op_create", ctypes.byref(h), self.DFTdim, sizes, int(self.rdft), prec, mp,
                       int(miss.size), _stream())
         self._h = h
         n = ctypes.c_int64()
@@ -153,9 +153,11 @@ def M_perpt_M_perp_vec(op: FFTOperator, vec):
     return out
 
 
-def beta_to_DFT(op: FFTOperator, beta, full=False):
+def beta_to_DFT(op: FFTOperator, beta, full=None):
     """beta -> spectrum (complex128 tensor, Julia layout flattened);
-    half spectrum (M1+1, N2, N3) unless full (fft_utils.jl:195-222)."""
+    half spectrum (M1+1, N2, N3) for an rdft operator, else the full one
+    (fft_utils.jl:195-222); `full` overrides."""
+    full = (not op.rdft) if full is None else full
     beta = op._vec(beta, dtype=torch.float64)
     rows = op.DFTsize[0] if full else op.DFTsize[0] // 2 + 1
     n = rows * (op.N // op.DFTsize[0])
@@ -164,8 +166,10 @@ def beta_to_DFT(op: FFTOperator, beta, full=False):
     return torch.view_as_complex(out.view(n, 2))
 
 
-def DFT_to_beta(op: FFTOperator, v, full=False):
-    """spectrum -> beta (fft_utils.jl:144-174); v complex, Julia layout."""
+def DFT_to_beta(op: FFTOperator, v, full=None):
+    """spectrum -> beta (fft_utils.jl:144-174); v complex, Julia layout, the
+    half spectrum for an rdft operator, else the full one; `full` overrides."""
+    full = (not op.rdft) if full is None else full
     rows = op.DFTsize[0] if full else op.DFTsize[0] // 2 + 1
     n = rows * (op.N // op.DFTsize[0])
     if not isinstance(v, torch.Tensor):

@@ -27,7 +27,7 @@ class FFTNLPModel:
     """x = [beta; c], nvar = ncon = 2N, constraints -beta - c <= 0,
     beta - c <= 0 (fft_model.jl:20-34)."""
 
-    def __init__(self, parameters: FFTParameters, krylov_solver="cg", rdft=True,
+    def __init__(self, parameters: FFTParameters, krylov_solver="cg", rdft=False,
                  preconditioner=True, dtype=torch.float64, device="cuda", mixed=False):
         if krylov_solver != "cg":
             raise NotImplementedError("only krylov_solver='cg' is implemented")

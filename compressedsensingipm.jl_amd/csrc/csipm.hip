@@ -104,6 +104,7 @@ struct csipm_op {
   bool bs[3] = {false, false, false};
   // 1D lines longer than one workgroup holds: four-step over B rows x A
   // columns of the length-F transform (F = N, or Bluestein's M when obs)
+  int rdft = 1;                  // as created (0: the complex path's full-spectrum maps)
   bool big = false, obs = false;
   int bA = 0, bB = 0, bF = 0;
   FftPlan planA{}, planB{};
@@ -981,7 +982,11 @@ static int create_impl(csipm_op* h, int dim, const int64_t sizes[3], int rdft, i
                        const int64_t* missing, int64_t n_missing, void* stream) {
   if (dim < 1 || dim > 3) return fail(CSIPM_ERR_ARG, "dim must be 1, 2 or 3 (got %d)", dim);
   if (!sizes) return fail(CSIPM_ERR_ARG, "sizes is null");
-  if (rdft != 1) return fail(CSIPM_ERR_UNSUPPORTED, "only the rdft=true path is implemented");
+  // rdft = 0 (the reference's complex path) computes the same operator: the
+  // complex transforms of a Hermitian spectrum and of real samples equal the
+  // C2R / R2C ones, so both run the real-to-complex kernels
+  if (rdft != 0 && rdft != 1) return fail(CSIPM_ERR_ARG, "rdft must be 0 or 1");
+  h->rdft = rdft;
   if (precision != CSIPM_F64 && precision != CSIPM_F32 && precision != CSIPM_MIXED)
     return fail(CSIPM_ERR_ARG, "bad precision %d", precision);
   h->dim = dim;

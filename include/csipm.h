@@ -55,7 +55,11 @@ const char* csipm_last_error(void);
  * index_missing, rdft) and its plan_rfft (src/fft_utils.jl:16-54).
  *   dim        1, 2 or 3 (FFTParameters.DFTdim, src/fft_utils.jl:1-14)
  *   sizes      N1, N2, N3 (unused trailing entries ignored)
- *   rdft       must be 1 (the real-to-complex path); 0 is reserved
+ *   rdft       1: the real-to-complex path; 0: the complex path (the
+ *              reference's default).  Both compute the same operator (the
+ *              complex FFT of a Hermitian spectrum / of real samples equals
+ *              the C2R / R2C one) with the same kernels; rdft only selects
+ *              the spectrum form the maps use by default (half or full)
  *   precision  CSIPM_F64, CSIPM_F32 or CSIPM_MIXED (see above)
  *   missing    HOST array of n_missing 0-based linear indices (index_missing
  *              of FFTParameters, converted from Julia's 1-based Int), may be

@@ -1,6 +1,6 @@
 """CPU FP64 oracle for the FFT operator M (TEST INFRASTRUCTURE ONLY).
 
-Restates src/fft_utils.jl (rdft path):
+Restates src/fft_utils.jl (rdft path, and with rdft=False the complex path):
 
 * M_perp_beta  (:95-117)  y = mask .* (C2R(beta_to_DFT(beta)) / N * sqrt(N))
 * M_perpt_z    (:73-93)   beta = DFT_to_beta(R2C(z) / sqrt(N))
@@ -35,8 +35,16 @@ def keep_mask(dims, missing):
     return keep
 
 
-def m_perp_beta(beta, dims, missing=None, workers=1):
+def m_perp_beta(beta, dims, missing=None, workers=1, rdft=True):
+    """rdft=False restates the complex path (fft_utils.jl:109-110): the full
+    Hermitian spectrum, a complex inverse FFT, the real part."""
     n = int(np.prod(dims))
+    if not rdft:
+        v = maps.beta_to_dft(beta, dims, rdft=False)
+        x = sfft.ifftn(v.reshape(_cshape(dims)), workers=workers).real.reshape(-1) * np.sqrt(n)
+        if missing is not None and len(missing):
+            x[np.asarray(missing, dtype=np.int64)] = 0.0
+        return x
     v = maps.beta_to_dft(beta, dims, rdft=True)
     vc = v.reshape(_cshape(maps.spectrum_shape(dims, True)))
     x = sfft.irfftn(vc, s=_cshape(dims), workers=workers).reshape(-1)
@@ -46,15 +54,20 @@ def m_perp_beta(beta, dims, missing=None, workers=1):
     return x
 
 
-def m_perpt_z(z, dims, workers=1):
+def m_perpt_z(z, dims, workers=1, rdft=True):
+    """rdft=False: complex forward FFT of z (fft_utils.jl:80-81), the map
+    reading the full spectrum."""
     n = int(np.prod(dims))
     zc = np.asarray(z, dtype=np.float64).reshape(_cshape(dims))
+    if not rdft:
+        v = sfft.fftn(zc.astype(np.complex128), workers=workers) / np.sqrt(n)
+        return maps.dft_to_beta(v.reshape(-1), dims)
     v = sfft.rfftn(zc, workers=workers) / np.sqrt(n)
     return maps.dft_to_beta(v.reshape(-1), dims)
 
 
-def mtm(beta, dims, missing=None, workers=1):
-    return m_perpt_z(m_perp_beta(beta, dims, missing, workers), dims, workers)
+def mtm(beta, dims, missing=None, workers=1, rdft=True):
+    return m_perpt_z(m_perp_beta(beta, dims, missing, workers, rdft), dims, workers, rdft)
 
 
 # independent statement (test/fft_wei.jl:1-19): full complex FFTs + block maps

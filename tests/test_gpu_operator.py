@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 import torch
 
-from oracle import kkt
+from oracle import kkt, maps
 from oracle import operator as op
 
 pytestmark = pytest.mark.gpu
@@ -157,3 +157,26 @@ def test_cg_solve(cuda, dims):
     assert abs(it.value - itr) <= 1
     assert res.value <= 1e-12
     assert _rel(x.cpu().numpy(), xr) <= 1e-10
+
+
+@pytest.mark.parametrize("dims", [(1000,), (16, 12), (15, 17), (9, 10, 7), (20000,)])
+def test_complex_path_operator(cuda, dims):
+    """An rdft=False handle (the reference's default) against the oracle's
+    complex path; its maps default to the full spectrum."""
+    from csipm.fft_operator import (FFTOperator, M_perp_beta, M_perpt_M_perp_vec, M_perpt_z,
+                                    beta_to_DFT, DFT_to_beta)
+    print("complex path", dims, flush=True)
+    n = int(np.prod(dims))
+    rng = np.random.default_rng(n + 2)
+    miss = np.sort(rng.choice(n, size=n // 8, replace=False))
+    o = FFTOperator(n, len(dims), dims, miss, rdft=False)
+    x = rng.standard_normal(n)
+    xt = torch.from_numpy(x).to(cuda)
+    assert _rel(M_perpt_M_perp_vec(o, xt).cpu().numpy(), op.mtm(x, dims, miss, rdft=False)) <= TOL64
+    assert _rel(M_perp_beta(o, xt).cpu().numpy(), op.m_perp_beta(x, dims, miss, rdft=False)) <= TOL64
+    assert _rel(M_perpt_z(o, xt).cpu().numpy(), op.m_perpt_z(x, dims, rdft=False)) <= TOL64
+    v = beta_to_DFT(o, xt).cpu().numpy()
+    assert v.size == n
+    assert np.array_equal(v, maps.beta_to_dft(x, dims, rdft=False))
+    assert np.array_equal(DFT_to_beta(o, torch.from_numpy(v).to(cuda)).cpu().numpy(),
+                          maps.dft_to_beta(v, dims))

@@ -142,3 +142,22 @@ def test_solve_vs_oracle(cuda, case, mixed):
     both = clear(ref.beta) & clear(beta)
     assert np.array_equal(sg[both], sr[both])
     assert both.mean() > 0.99
+
+
+def test_default_model_is_complex_path(cuda):
+    """FFTNLPModel defaults to rdft=False like the reference (fft_model.jl:14);
+    the solve reaches the oracle objective within 1e-6."""
+    import csipm.workloads as wl
+    from csipm.fft_operator import FFTParameters
+    from csipm.model import FFTNLPModel, ipm_solve
+    p = wl.batch_instance((63, 63), seed=31)
+    nlp = FFTNLPModel(FFTParameters(2, p.dims, p.z0, p.lam, p.missing))
+    assert nlp.op_fft.rdft is False
+    st = ipm_solve(nlp, tol=1e-8)
+    b = op.m_perpt_z(p.z0, p.dims)
+    mt = lambda v: op.mtm(v, p.dims, p.missing)    # noqa: E731
+    ref = ipm.solve(b, p.lam, mt, ipm.IPMOptions(tol=1e-10, cg_rtol=1e-13))
+    beta = st.solution.cpu().numpy()
+    f = ipm.objective(beta, np.abs(beta), mt(beta), b, p.lam)
+    f_ref = ipm.objective(ref.beta, np.abs(ref.beta), mt(ref.beta), b, p.lam)
+    assert st.status == "solved" and abs(f - f_ref) <= 1e-6 * abs(f_ref)

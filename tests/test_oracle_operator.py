@@ -179,3 +179,22 @@ def test_ipm_planted_operator_mistake():
     fp, gap, primal = ipm.lasso_certificate(r.beta, b, p.lam, right,
                                             lambda v: op.m_perp_beta(v, dims, p.missing), p.z0)
     assert gap < 1e-8 * abs(primal)
+
+
+@pytest.mark.parametrize("dims", [(100,), (16, 12), (9, 10, 7), (15, 17), (8, 6, 4)])
+def test_complex_path_equals_real_path(dims):
+    """rdft=False (fft_utils.jl:80-81,109-110: complex FFTs over the full
+    Hermitian spectrum) against the real-to-complex statement and the
+    independent full-spectrum block maps of test/fft_wei.jl."""
+    n = int(np.prod(dims))
+    rng = np.random.default_rng(n + 1)
+    x = rng.standard_normal(n)
+    miss = np.sort(rng.choice(n, size=n // 6, replace=False))
+    c = op.mtm(x, dims, miss, rdft=False)
+    assert np.abs(c - op.mtm(x, dims, miss)).max() <= 1e-13 * np.abs(c).max()
+    if all(d % 2 == 0 for d in dims):
+        assert np.abs(c - op.mtm_wei(x, dims, miss)).max() <= 1e-13 * np.abs(c).max()
+    y = op.m_perp_beta(x, dims, miss, rdft=False)
+    assert np.abs(y - op.m_perp_beta(x, dims, miss)).max() <= 1e-13 * np.abs(y).max()
+    b = op.m_perpt_z(x, dims, rdft=False)
+    assert np.abs(b - op.m_perpt_z(x, dims)).max() <= 1e-13 * np.abs(b).max()
