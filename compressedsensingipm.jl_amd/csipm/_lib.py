@@ -40,6 +40,7 @@ SIGNATURES = {
     "csipm_cg_solve": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _d, _i64,
                             ctypes.POINTER(_i64), ctypes.POINTER(_d), ctypes.POINTER(_i)]),
     "csipm_solve": (_i, [_vp, _vp, _d, _d, _i64, _d, _i64, _vp, ctypes.POINTER(_d)]),
+    "csipm_admm": (_i, [_vp, _vp, _d, _d, _i64, _d, _vp, ctypes.POINTER(_d)]),
     "csipm_batch_create": (_i, [ctypes.POINTER(_vp), _i, _i64, _i, _vp]),
     "csipm_batch_destroy": (_i, [_vp]),
     "csipm_batch_slots": (_i, [_vp, ctypes.POINTER(_i64)]),

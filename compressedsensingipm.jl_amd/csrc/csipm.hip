@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <initializer_list>
+#include <limits>
 #include <memory>
 #include <mutex>
 #include <unordered_map>
@@ -860,6 +861,33 @@ int cg_solve_solo(csipm_op* h, const T* b, T* x, const T* L1, const T* L2, const
   return engine_leave(E);
 }
 
+// csipm_admm: ADMM on a batch of one
+template <typename TC>
+int admm_solo(csipm_op* h, const double* z0, double lam, double rho, int64_t maxt, double tol,
+              double* beta_out, double* info) {
+  const auto t0 = std::chrono::steady_clock::now();
+  Engine* E = nullptr;
+  CHK(solo_engine(h, &E));
+  int id = -1;
+  CHK(engine_admit(E, h, z0, lam, beta_out, 2 * h->N, &id));   // Krylov cg: itmax = 2n
+  const double eps = std::sqrt(std::numeric_limits<double>::epsilon());
+  const int mt = (int)std::min<int64_t>(maxt, 1 << 30);
+  int rc = CSIPM_OK;
+  for (;;) {
+    int running = 0;
+    if ((rc = engine_admm_step<TC>(E, rho, mt, tol, eps, eps, &running))) break;
+    if (running == 0) break;
+  }
+  if (rc) {
+    engine_drop(E, id);
+    return rc;
+  }
+  CHK(engine_harvest(E, id, info));
+  info[0] = 0.0;
+  info[6] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return engine_leave(E);
+}
+
 // csipm_solve: the interior-point driver on a batch of one
 template <typename TC>
 int solve_solo(csipm_op* h, const double* z0, double lam, double tol, int64_t max_iter,
@@ -1230,6 +1258,18 @@ int csipm_solve(csipm_op_t h, const double* z0, double lambda, double tol, int64
   if (h->prec == CSIPM_MIXED)
     return solve_solo<float>(h, z0, lambda, tol, max_iter, cg_rtol, cg_itmax, beta_out, info);
   return fail(CSIPM_ERR_UNSUPPORTED, "csipm_solve needs a CSIPM_F64 or CSIPM_MIXED handle");
+}
+
+int csipm_admm(csipm_op_t h, const double* z0, double lambda, double rho, int64_t maxt,
+               double tol, double* beta_out, double* info) {
+  if (!h || !z0 || !beta_out || !info) return fail(CSIPM_ERR_ARG, "null argument");
+  if (!(lambda >= 0) || !(rho > 0) || !(tol > 0) || maxt < 1)
+    return fail(CSIPM_ERR_ARG, "bad lambda / rho / tol / maxt");
+  if (h->prec == CSIPM_F64)
+    return admm_solo<double>(h, z0, lambda, rho, maxt, tol, beta_out, info);
+  if (h->prec == CSIPM_MIXED)
+    return admm_solo<float>(h, z0, lambda, rho, maxt, tol, beta_out, info);
+  return fail(CSIPM_ERR_UNSUPPORTED, "csipm_admm needs a CSIPM_F64 or CSIPM_MIXED handle");
 }
 
 #define DISPATCH_T(h, F64EXPR, F32EXPR) (OP_DOUBLE(h) ? (F64EXPR) : (F32EXPR))

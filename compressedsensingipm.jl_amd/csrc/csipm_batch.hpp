@@ -703,6 +703,110 @@ __global__ __launch_bounds__(256) void bk_ipm_update(
   }
 }
 
+// ===========================================================================
+// ADMM (src/admm.jl:8-61) over a batch, in the spectrum order: b0 = M^T z0,
+// x0 (previous x), z, y in FP64; each iteration solves
+// (M^T M + rho I) x = b0 + rho z - y by CG (the reduced-system apply with
+// D = rho, no preconditioner), then z = S_{lambda/rho}(x + y/rho),
+// y += rho (x - z), err = max(||x - x0||, ||y - y0||, ||z - z0||).  The soft
+// threshold is odd, so it commutes with the signed permutation into u.
+// ===========================================================================
+__global__ __launch_bounds__(256) void k_fill(double* x, int64_t n, double v) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    x[i] = v;
+}
+
+template <typename TC>
+__global__ __launch_bounds__(256) void bk_admm_setup(BSeg seg, const BInst* __restrict__ info,
+                                                     const BFlags* flags, const double* x0,
+                                                     double* z, double* y, TC* D, TC* P,
+                                                     double rho, const uint32_t* valid) {
+  BSEG_BEGIN
+  BFOR(i) {
+    const bool ok = BVALID(valid, i);
+    z[i] = x0[i];
+    y[i] = 0.0;
+    D[i] = (TC)(ok ? rho : 0.0);
+    P[i] = (TC)1;
+  }
+}
+
+// b = b0 + rho z - y (first block of the CG right-hand side)
+template <typename TC>
+__global__ __launch_bounds__(256) void bk_admm_rhs(BSeg seg, const BInst* __restrict__ info,
+                                                   const BFlags* flags,
+                                                   const double* __restrict__ b0,
+                                                   const double* __restrict__ z,
+                                                   const double* __restrict__ y, double rho,
+                                                   TC* rhs, const uint32_t* valid) {
+  BSEG_BEGIN
+  BFOR(i) rhs[i + I.off] = (TC)(BVALID(valid, i) ? b0[i] + rho * z[i] - y[i] : 0.0);
+}
+
+__device__ __forceinline__ double soft_threshold(double x, double t) {
+  return x > t ? x - t : (x < -t ? x + t : 0.0);
+}
+
+// x from CG; z, y updates in place; partials of ||x - x0||^2, ||y - y0||^2, ||z - z0||^2
+template <typename TC>
+__global__ __launch_bounds__(256) void bk_admm_update(BSeg seg, const BInst* __restrict__ info,
+                                                      const BFlags* flags,
+                                                      const TC* __restrict__ xs, double* x0,
+                                                      double* z, double* y, double rho,
+                                                      double* part, const uint32_t* valid) {
+  BSEG_BEGIN
+  const double thr = I.lam / rho;
+  double a[3] = {0, 0, 0};
+  BFOR(i) {
+    if (!BVALID(valid, i)) continue;
+    const double x1 = (double)xs[i + I.off];
+    const double yo = y[i], zo = z[i];
+    const double z1 = soft_threshold(x1 + yo / rho, thr);
+    const double y1 = yo + rho * (x1 - z1);
+    const double dx = x1 - x0[i], dy = y1 - yo, dz = z1 - zo;
+    a[0] += dx * dx;
+    a[1] += dy * dy;
+    a[2] += dz * dz;
+    x0[i] = x1;
+    z[i] = z1;
+    y[i] = y1;
+  }
+  const int ops[3] = {0, 0, 0};
+  bstore_partials<3>(part + I.pbase, a, ops, lb, nbk);
+}
+
+// one block per instance: err, the iteration count and the stopping test
+// (admm.jl:32: while t < maxt and err > tol)
+__global__ void bk_admm_finalize(BSeg vseg, const int* ids, const BInst* __restrict__ info,
+                                 BFlags* flags, const double* part, double tol, int maxt) {
+  const int inst = ids[blockIdx.x];
+  if (flags[inst].done) return;
+  const BInst& I = info[inst];
+  const int np = vseg.first[inst + 1] - vseg.first[inst];
+  double e[3];
+  for (int q = 0; q < 3; ++q) {
+    e[q] = sum_partials(part + I.pbase + q * np, np);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    BFlags& fl = flags[inst];
+    const double err = fmax(sqrt(e[0]), fmax(sqrt(e[1]), sqrt(e[2])));
+    fl.iters += 1;
+    fl.dual = err;
+    if (!isfinite(err)) {
+      fl.status = 2;
+      fl.done = 1;
+    } else if (err <= tol) {
+      fl.status = 0;
+      fl.done = 1;
+    } else if (fl.iters >= maxt) {
+      fl.status = 1;
+      fl.done = 1;
+    }
+  }
+}
+
 #undef BSEG_BEGIN
 #undef BFOR
 #undef BVALID

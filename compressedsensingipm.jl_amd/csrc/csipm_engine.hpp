@@ -179,7 +179,8 @@ int run_chain(const Chain& c, hipStream_t st) {
 // the engine
 // ---------------------------------------------------------------------------
 constexpr int64_t kSlotAlign = 256;   // slot ranges start on 256-slot boundaries
-constexpr int kPartPerAlign = 16;     // partial-sum slots per 256 vector slots
+constexpr int kPartPerAlign = 64;     // partial-sum slots per 256 vector slots (a line pass
+                                      // over U slots has at most U/4 workgroups)
 
 // FP64 control vectors (one block, pool length) and two-block FP64 vectors
 enum { V_B = 0, V_BETA, V_C, V_Z1, V_Z2, V_QB, V_S1, V_S2, V_DT1, V_DT2, V_DZ1, V_DZ2, V_L1D,
@@ -1070,5 +1071,79 @@ int engine_apply_timed(Engine* E, int64_t nrep, double* ms, double* bytes, std::
                nm[sb.pass];
   }
   *nsub = ns;
+  return CSIPM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// ADMM (src/admm.jl:8-61) on the engine's instances.  State: b0 = M^T z0
+// (V_B), x0 (V_C), z (V_BETA, the result), y (V_Z1); the x-update CG runs on
+// the reduced system with D = rho (C_L1) and no preconditioner (C_P11 = 1),
+// Krylov.jl's cg defaults: x from 0, ||r|| <= atol + rtol ||b||, 2N iterations.
+// Initial point (admm.jl:19-24): x0 = z = ones (signed into u), y = 0.
+// ---------------------------------------------------------------------------
+template <typename TC>
+int engine_admm_init_fresh(Engine* E, double rho) {
+  std::vector<int> fresh;
+  for (int id : E->active)
+    if (E->inst[(size_t)id].fresh) fresh.push_back(id);
+  if (fresh.empty()) return CSIPM_OK;
+  Chain mtz;
+  int* mem = nullptr;
+  BSeg seg{};
+  const int* ids = nullptr;
+  int grid = 0;
+  int rc = CSIPM_OK;
+  do {
+    if ((rc = build_chain<double>(E, fresh, CALL_MTZ, 1, mtz))) break;
+    if ((rc = upload_vseg(E, fresh, &mem, &seg, &ids, &grid))) break;
+    if ((rc = run_chain<double>(mtz, E->stream))) break;
+    for (int id : fresh) {
+      Engine::Inst& in = E->inst[(size_t)id];
+      double* ones = E->W(V_DT1) + in.off;   // beta-layout scratch (N <= U slots)
+      hipLaunchKernelGGL(k_fill, dim3(grid_for(in.h->N, 4)), dim3(kNT), 0, E->stream, ones,
+                         in.h->N, 1.0);
+      if ((rc = to_u<double>(in.h, ones, E->W(V_C) + in.off, E->stream))) break;
+    }
+    if (rc) break;
+    hipLaunchKernelGGL((bk_admm_setup<TC>), dim3(grid), dim3(kNT), 0, E->stream, seg, E->info,
+                       E->flags, E->W(V_C), E->W(V_BETA), E->W(V_Z1), E->C<TC>(C_L1),
+                       E->C<TC>(C_P11), rho, E->valid);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(E->stream) != hipSuccess)
+      rc = fail(CSIPM_ERR_HIP, "ADMM initialisation failed");
+  } while (false);
+  mtz.release();
+  if (mem) (void)hipFree(mem);
+  if (rc) return rc;
+  for (int id : fresh) E->inst[(size_t)id].fresh = false;
+  return CSIPM_OK;
+}
+
+// one ADMM iteration of every running instance; *running: those still iterating
+template <typename TC>
+int engine_admm_step(Engine* E, double rho, int maxt, double tol, double cg_atol,
+                     double cg_rtol, int* running) {
+  StatTimer tm(&E->stats[Engine::ST_STEP]);
+  E->stats[Engine::ST_NSTEP] += 1;
+  CHK(engine_admm_init_fresh<TC>(E, rho));
+  CHK(engine_tables<TC>(E, 1, false));
+  const int na = (int)E->active.size();
+  *running = 0;
+  if (na == 0) return CSIPM_OK;
+  hipStream_t st = E->stream;
+  hipLaunchKernelGGL((bk_admm_rhs<TC>), dim3(E->vgrid), dim3(kNT), 0, st, E->vseg, E->info,
+                     E->flags, E->W(V_B), E->W(V_BETA), E->W(V_Z1), rho, E->C2<TC>(C2_RHS),
+                     E->valid);
+  HIPCHK(hipGetLastError());
+  CHK(engine_cg<TC>(E, 1, cg_atol, cg_rtol, 0));
+  hipLaunchKernelGGL((bk_admm_update<TC>), dim3(E->vgrid), dim3(kNT), 0, st, E->vseg, E->info,
+                     E->flags, E->C2<TC>(C2_D), E->W(V_C), E->W(V_BETA), E->W(V_Z1), rho,
+                     E->part, E->valid);
+  hipLaunchKernelGGL(bk_admm_finalize, dim3(na), dim3(kNT), 0, st, E->vseg, E->ids, E->info,
+                     E->flags, E->part, tol, maxt);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(E->flags_h, E->flags, sizeof(BFlags) * (size_t)E->maxinst,
+                        hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  for (int id : E->active) *running += E->flags_h[id].done ? 0 : 1;
   return CSIPM_OK;
 }

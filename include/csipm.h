@@ -174,6 +174,17 @@ int csipm_solve(csipm_op_t h, const double* z0, double lambda, double tol,
                 int64_t max_iter, double cg_rtol, int64_t cg_itmax,
                 double* beta_out, double* info);
 
+/* ADMM for the same problem (CompressedSensingADMM, src/admm.jl:8-61):
+ * x-update by CG on (M^T M + rho I) (Krylov.jl cg defaults: atol = rtol =
+ * sqrt(eps), 2N iterations), z = soft threshold of x + y/rho at lambda/rho,
+ * y += rho (x - z); stops when max(||dx||, ||dy||, ||dz||) <= tol or after
+ * maxt iterations.  Returns z in beta_out (device, N doubles); info[8] =
+ * [0, iterations, total CG iterations, status (0 converged, 1 maxt,
+ * 2 failure), last err, 0, seconds, 0].  Handle precision CSIPM_F64 (the
+ * reference's arithmetic) or CSIPM_MIXED (FP32 CG).  Synchronous.        */
+int csipm_admm(csipm_op_t h, const double* z0, double lambda, double rho,
+               int64_t maxt, double tol, double* beta_out, double* info);
+
 /* ------------------------------------------------------------------------
  * Batch engine: many solves of uneven sizes and dimensionalities in the same
  * kernel launches (one launch per pass / vector step for all instances),

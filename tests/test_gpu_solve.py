@@ -161,3 +161,46 @@ def test_default_model_is_complex_path(cuda):
     f = ipm.objective(beta, np.abs(beta), mt(beta), b, p.lam)
     f_ref = ipm.objective(ref.beta, np.abs(ref.beta), mt(ref.beta), b, p.lam)
     assert st.status == "solved" and abs(f - f_ref) <= 1e-6 * abs(f_ref)
+
+
+@pytest.mark.parametrize("dims", [(100,), (200,), (500,), (16, 16), (12, 18), (16, 24),
+                                  (32, 32), (8, 8, 8), (2, 4, 6), (6, 10, 12), (14, 8, 4),
+                                  (8, 6, 4), (16, 16, 16)])
+def test_admm_known_answer(cuda, dims):
+    """test/unit_tests_admm.jl: every shape, check=true (lambda = 0, no mask,
+    rho = 1, tol 1e-8) -> beta = DFT_to_beta(fft(x)/sqrt(N)) within 1e-6."""
+    import csipm.workloads as wl
+    from csipm.admm import CompressedSensingADMM
+    from csipm.fft_operator import FFTOperator, FFTParameters
+    print("admm known answer", dims, flush=True)
+    p = wl.reference_example(dims, check=True)
+    params = FFTParameters(len(dims), dims, p.z0, 0.0, p.missing)
+    st = CompressedSensingADMM(params, FFTOperator(p.N, len(dims), dims, p.missing), rho=1.0,
+                               maxt=1000, tol=1e-8, stats=True)
+    assert st.status == "converged"
+    vh = np.fft.rfftn(p.x_true.reshape(dims[::-1])).reshape(-1) / np.sqrt(p.N)
+    assert np.linalg.norm(maps.dft_to_beta(vh, dims) - st.solution.cpu().numpy()) <= 1e-6
+
+
+@pytest.mark.parametrize("case", ["1d", "2d", "3d"])
+def test_admm_vs_oracle(cuda, case):
+    """Masked problems with lambda > 0: the device ADMM follows the oracle's
+    iteration (same count within 1) to the same point (1e-8 relative)."""
+    import csipm.workloads as wl
+    from csipm.admm import CompressedSensingADMM
+    from oracle import admm
+    from csipm.fft_operator import FFTOperator, FFTParameters
+    print("admm vs oracle", case, flush=True)
+    p = {"1d": lambda: wl.batch_instance((300,), seed=41),
+         "2d": lambda: wl.batch_instance((24, 20), seed=42),
+         "3d": lambda: wl.batch_instance((10, 8, 12), seed=43)}[case]()
+    b = op.m_perpt_z(p.z0, p.dims)
+    mt = lambda v: op.mtm(v, p.dims, p.missing)    # noqa: E731
+    ref = admm.solve(b, p.lam, mt, rho=0.1, maxt=3000, tol=1e-8)
+    params = FFTParameters(len(p.dims), p.dims, p.z0, p.lam, p.missing)
+    st = CompressedSensingADMM(params, FFTOperator(p.N, len(p.dims), p.dims, p.missing),
+                               rho=0.1, maxt=3000, tol=1e-8, stats=True)
+    assert ref.converged and st.status == "converged"
+    assert abs(st.iterations - ref.iterations) <= 1
+    z = st.solution.cpu().numpy()
+    assert np.linalg.norm(z - ref.beta) <= 1e-8 * np.linalg.norm(ref.beta) + 1e-10

