@@ -18,6 +18,11 @@
 
 #include "csipm.h"
 #include "csipm_batch.hpp"
+// the FFT pass kernels are compiled in separate translation units when the
+// build provides their declarations
+#if __has_include("fft_extern.inc")
+#include "fft_extern.inc"
+#endif
 
 using namespace csipm;
 
@@ -306,24 +311,26 @@ static int upload_twiddles(int L, void** out) {
   return CSIPM_OK;
 }
 
+// smallest tile (lines per workgroup) whose rows span 64 bytes: with the
+// XCD-aware block order of the pass kernels, the two tiles sharing a 128-byte
+// line run on one XCD and fetch it once
+static int min_tile_lg(size_t cs) {
+  int lg = 0;
+  while ((size_t)(1 << lg) * cs < 64) ++lg;
+  return lg;
+}
+
 // line FFTs along dim 2/3: 2^lg lines of length L per workgroup, as many as
 // the registers (lines*L <= NT*E) and 64 KB of LDS allow, fewer when that
 // would leave the launch with under min_grid workgroups (1024 for one
 // instance; a batch shares the target among its instances)
-// lines per workgroup below which a row of the tile is under one 128-byte
-// cache line (the reads of a line pass then fetch twice what they use)
-static int min_tile_lg(size_t cs) {
-  int lg = 0;
-  while ((size_t)(1 << lg) * cs < 128) ++lg;
-  return lg;
-}
 
 static LineCfg line_cfg(int H1, int L, int64_t others, size_t cs, int min_grid = 1024) {
   LineCfg c{};
   int lg = 0;
   while ((2 << lg) * L <= kNT * kE && (size_t)(2 << lg) * L * cs <= kLdsSmall && (1 << lg) < H1)
     ++lg;
-  // below one cache line per row only while the launch would not even give a
+  // below 64-byte rows only while the launch would not even give a
   // quarter of the workgroup target (small single instances: latency first)
   const int lg_min = std::min(lg, min_tile_lg(cs));
   auto grid_of = [&](int l) { return (int64_t)((H1 + (1 << l) - 1) >> l) * others; };

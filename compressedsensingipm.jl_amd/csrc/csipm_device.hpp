@@ -439,6 +439,19 @@ __device__ __forceinline__ void stockham_stage(cpx<T>* __restrict__ lds, const F
   int tid = (int)threadIdx.x;
   asm volatile("" : "+v"(tid));
   cpx<T> v[NQ][R];
+  cpx<T> w[NQ][R];
+  // twiddle loads first (index clamped into the table for idle lanes): their
+  // latency overlaps the LDS reads and the barrier instead of following them
+  if (Ns > 1) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const int b = min(tid + q * NT, total - 1);
+      const int j = b >> lg;
+      const int step = (j - fdiv(j, nsm) * Ns) * tws;
+#pragma unroll
+      for (int r = 1; r < R; ++r) w[q][r] = tw[r * step];
+    }
+  }
 #pragma unroll
   for (int q = 0; q < NQ; ++q) {
     const int b = tid + q * NT;
@@ -460,12 +473,11 @@ __device__ __forceinline__ void stockham_stage(cpx<T>* __restrict__ lds, const F
       int k = 0;
       if (Ns > 1) {
         k = j - fdiv(j, nsm) * Ns;
-        const int step = k * tws;
 #pragma unroll
         for (int r = 1; r < R; ++r) {
-          cpx<T> w = tw[r * step];
-          if (!FWD) w.y = -w.y;
-          v[q][r] = cmul(v[q][r], w);
+          cpx<T> t = w[q][r];
+          if (!FWD) t.y = -t.y;
+          v[q][r] = cmul(v[q][r], t);
         }
       }
       cpx<T>* dst = lds + ((j - k) * R + k) * ps + line;
@@ -535,6 +547,41 @@ __host__ __device__ __forceinline__ cpx<T> u_to_v(const cpx<T>* __restrict__ u, 
   const int m3 = k3 == 0 ? 0 : g.n[2] - k3;
   const cpx<T> w = u[k1 + (int64_t)H1 * (m2 + (int64_t)g.n[1] * m3)];
   return {w.x / sq2, -(w.y / sq2)};
+}
+
+// u_to_v in two steps, without branches around the load: u_src returns the
+// slot u_to_v reads and how its value maps (0: w/sqrt2, 1: (w.x, 0),
+// 2: conj(w)/sqrt2); u_map applies that map with u_to_v's arithmetic.
+template <int D>
+__host__ __device__ __forceinline__ int64_t u_src(int k1, int k2, int k3, const MapGeom& g,
+                                                  int H1, int& mode) {
+  const Cat c1 = cat_of(k1, g, 0);
+  const Cat c2 = D >= 2 ? cat_of(k2, g, 1) : Cat{0, 0};
+  const Cat c3 = D >= 3 ? cat_of(k3, g, 2) : Cat{0, 0};
+  const bool rep = c1.c == 1 || is_rep<D>(c1, c2, c3);
+  const bool vert = c1.c != 1 && rep && c2.c == 0 && c3.c == 0;
+  const int m2 = (rep || k2 == 0) ? k2 : g.n[1] - k2;
+  const int m3 = (rep || k3 == 0) ? k3 : g.n[2] - k3;
+  mode = vert ? 1 : (rep ? 0 : 2);
+  return k1 + (int64_t)H1 * (m2 + (int64_t)g.n[1] * m3);
+}
+
+template <typename T>
+__host__ __device__ __forceinline__ cpx<T> u_map(cpx<T> w, int mode) {
+#pragma clang fp contract(off)
+  const T sq2 = (T)1.4142135623730951;
+  const T x = w.x / sq2, y = w.y / sq2;
+  if (mode == 1) return {w.x, (T)0};
+  return {x, mode == 2 ? -y : y};
+}
+
+// sqrt2 * v, v_to_u of a representative element (kept apart from any
+// following add, as in v_to_u)
+template <typename T>
+__host__ __device__ __forceinline__ cpx<T> scale_sq2(cpx<T> v) {
+#pragma clang fp contract(off)
+  const T sq2 = (T)1.4142135623730951;
+  return {sq2 * v.x, sq2 * v.y};
 }
 
 // u slot value of spectrum element (k1,k2,k3) holding v

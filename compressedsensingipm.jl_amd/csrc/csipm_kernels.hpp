@@ -257,6 +257,86 @@ __device__ __forceinline__ double epi_apply_u(const Epi<T>& e, int64_t el, cpx<T
   return (double)xb.x * qb.x + (double)xb.y * qb.y + (double)xz.x * qz.x + (double)xz.y * qz.y;
 }
 
+// The epilogue of a u-space element in two steps, so that a thread can issue
+// the operand loads of several elements before it consumes any of them:
+// epi_fetch_u loads what kind `e.kind` reads (always in range for el < U),
+// epi_commit_u forms and stores the result and returns its x'y term.
+template <typename T>
+struct EpiOps {
+  cpx<T> xb, xz, l1, l2, ob, oz;
+};
+
+template <typename T>
+__device__ __forceinline__ EpiOps<T> epi_fetch_u(const Epi<T>& e, int64_t el) {
+  EpiOps<T> o;
+  const cpx<T> z = {(T)0, (T)0};
+  o.xb = o.xz = o.l1 = o.l2 = o.ob = o.oz = z;
+  if (e.kind == 0) return o;
+  o.xb = reinterpret_cast<const cpx<T>*>(e.xb)[el];
+  o.l1 = reinterpret_cast<const cpx<T>*>(e.L1)[el];
+  if (e.beta != (T)0) o.ob = reinterpret_cast<const cpx<T>*>(e.yb)[el];
+  if (e.kind == 2) return o;
+  o.xz = reinterpret_cast<const cpx<T>*>(e.xz)[el];
+  o.l2 = reinterpret_cast<const cpx<T>*>(e.L2)[el];
+  if (e.beta != (T)0) o.oz = reinterpret_cast<const cpx<T>*>(e.yz)[el];
+  return o;
+}
+
+// the arithmetic of epi_apply_u, on fetched operands
+template <typename T>
+__device__ __forceinline__ double epi_commit_u(const Epi<T>& e, int64_t el, cpx<T> m,
+                                               const EpiOps<T>& o) {
+  if (e.kind == 0) {
+    reinterpret_cast<cpx<T>*>(e.out)[el] = m;
+    return 0.0;
+  }
+  if (e.kind == 2) {
+    cpx<T> qb = {e.alpha * (m.x + o.l1.x * o.xb.x), e.alpha * (m.y + o.l1.y * o.xb.y)};
+    if (e.beta != (T)0) {
+      qb.x += e.beta * o.ob.x;
+      qb.y += e.beta * o.ob.y;
+    }
+    reinterpret_cast<cpx<T>*>(e.yb)[el] = qb;
+    return (double)o.xb.x * qb.x + (double)o.xb.y * qb.y;
+  }
+  cpx<T> qb = {e.alpha * (m.x + o.l1.x * o.xb.x + o.l2.x * o.xz.x),
+               e.alpha * (m.y + o.l1.y * o.xb.y + o.l2.y * o.xz.y)};
+  cpx<T> qz = {e.alpha * (o.l2.x * o.xb.x + o.l1.x * o.xz.x),
+               e.alpha * (o.l2.y * o.xb.y + o.l1.y * o.xz.y)};
+  if (e.beta != (T)0) {
+    qb.x += e.beta * o.ob.x;
+    qb.y += e.beta * o.ob.y;
+    qz.x += e.beta * o.oz.x;
+    qz.y += e.beta * o.oz.y;
+  }
+  reinterpret_cast<cpx<T>*>(e.yb)[el] = qb;
+  reinterpret_cast<cpx<T>*>(e.yz)[el] = qz;
+  return (double)o.xb.x * qb.x + (double)o.xb.y * qb.y + (double)o.xz.x * qz.x +
+         (double)o.xz.y * qz.y;
+}
+
+// Global-memory phases of the pass kernels run in groups of stage_k<T>()
+// elements per thread: fetch(i) issues the loads of element i (branch-free,
+// on an index clamped into range), put(i, r) consumes them.  All fetches of
+// a group come before its first put, so a thread has that many loads in
+// flight instead of one.
+template <typename T>
+constexpr int stage_k() {
+  return sizeof(T) == 4 ? 8 : 4;
+}
+
+template <int NT, int K, typename R, typename F, typename P>
+__device__ __forceinline__ void staged(int total, F&& fetch, P&& put) {
+  for (int i0 = threadIdx.x; i0 < total; i0 += K * NT) {
+    R r[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) r[j] = fetch(min(i0 + j * NT, total - 1));
+#pragma unroll
+    for (int j = 0; j < K; ++j)
+      if (i0 + j * NT < total) put(i0 + j * NT, r[j]);
+  }
+}
+
 // value v of half-spectrum element (k1,k2,k3) -> the beta entries reading it
 template <int D, typename T>
 __device__ __forceinline__ double scatter_elem(const Epi<T>& e, int k1, int k2, int k3,
@@ -388,12 +468,44 @@ __device__ __forceinline__ double spec_store(const PassArgs<T>& a, int k1, int k
   return epi_apply_u(a.epi, el, v_to_u<D, T>(v, k1, k2, k3, a.g));
 }
 
+// Workgroups are dispatched to the 8 XCDs round robin (block b on XCD b % 8),
+// each XCD with its own L2.  xcd_block() numbers the blocks so that blocks
+// b, b + 8, b + 16, .. (one XCD) take consecutive numbers: adjacent tiles,
+// which share the cache lines at their edges, then meet in one L2.  A
+// bijection of [0, gridDim.x); the last gridDim.x % 8 blocks keep their own.
+__device__ __forceinline__ int xcd_block() {
+  const int b = (int)blockIdx.x, q = (int)gridDim.x >> 3;
+  return b >= (q << 3) ? b : (b & 7) * q + (b >> 3);
+}
+
+// a fetched source value: mode -1 = outside the tile (zero), otherwise the
+// u_src map of an SRC_U load (other sources: the value as loaded)
+template <typename T>
+struct Raw {
+  cpx<T> w;
+  int mode;
+};
+
+template <typename T, int SRC>
+__device__ __forceinline__ cpx<T> raw_value(const Raw<T>& r) {
+  if (r.mode < 0) return {(T)0, (T)0};
+  return SRC == SRC_U ? u_map(r.w, r.mode) : r.w;
+}
+
+// a spectrum value read from LDS with its epilogue operands
+template <typename T>
+struct Fetched {
+  cpx<T> m;
+  EpiOps<T> ops;
+};
+
 // one workgroup's tile of a line pass (tile = its index within the pass)
 template <typename T, int NT, int E, int D, int SRC, int DST, bool FWD>
 __device__ __forceinline__ void lines_body(const PassArgs<T>& a, int tile) {
   extern __shared__ unsigned char smem_raw[];
   cpx<T>* lds = reinterpret_cast<cpx<T>*>(smem_raw);
-  if (a.done && *a.done) return;
+  // finished instances: the flag is read beside the loads, not before them
+  const int dflag = a.done ? *a.done : 0;
   const int L = a.len;
   const int lg = a.lg, nl = 1 << lg;
   const int tile1 = tile % a.ntile1;
@@ -404,35 +516,101 @@ __device__ __forceinline__ void lines_body(const PassArgs<T>& a, int tile) {
   const int64_t sl = a.axis == 1 ? s2 : s3;
   const int64_t base = k10 + (a.axis == 1 ? (int64_t)o * s3 : (int64_t)o * s2);
   const int total = L << lg;
-  for (int idx = threadIdx.x; idx < total; idx += NT) {
-    const int line = idx & (nl - 1);
-    const int el = idx >> lg;
-    cpx<T> val = {(T)0, (T)0};
-    if (line < nv) {
-      if (SRC == SRC_BUF) {
-        val = a.buf[base + line + el * sl];
-      } else {
-        const int k2 = a.axis == 1 ? el : o;
-        const int k3 = a.axis == 1 ? o : el;
-        val = spec_load<D, T, SRC>(a, k10 + line, k2, k3);
-      }
-    }
-    lds[idx] = val;
+  constexpr int K = stage_k<T>();
+  // a tile whose k1 are all representatives (1 <= k1 <= P1) reads and writes
+  // u at the scratch index with the plain sqrt2 scalings of u_to_v / v_to_u
+  const bool interior = k10 >= 1 && k10 + nv - 1 <= a.g.P[0];
+  if (SRC == SRC_U && interior) {
+    staged<NT, K, Raw<T>>(
+        total,
+        [&](int idx) {
+          const int line = idx & (nl - 1), el = idx >> lg;
+          Raw<T> r;
+          r.mode = line < nv ? 0 : -1;
+          r.w = a.u_in[base + (line < nv ? line : 0) + el * sl];
+          return r;
+        },
+        [&](int idx, const Raw<T>& r) { lds[idx] = raw_value<T, SRC>(r); });
+  } else {
+  staged<NT, K, Raw<T>>(
+      total,
+      [&](int idx) {
+        const int line = idx & (nl - 1), el = idx >> lg;
+        const bool ok = line < nv;
+        const int k1 = k10 + (ok ? line : 0);
+        Raw<T> r;
+        r.mode = 0;
+        if (SRC == SRC_BUF) {
+          r.w = a.buf[base + (k1 - k10) + el * sl];
+        } else {
+          const int k2 = a.axis == 1 ? el : o;
+          const int k3 = a.axis == 1 ? o : el;
+          if (SRC == SRC_U)
+            r.w = a.u_in[u_src<D>(k1, k2, k3, a.g, a.H1, r.mode)];
+          else
+            r.w = spec_load<D, T, SRC>(a, k1, k2, k3);
+        }
+        if (!ok) r.mode = -1;
+        return r;
+      },
+      [&](int idx, const Raw<T>& r) { lds[idx] = raw_value<T, SRC>(r); });
   }
+  if (dflag) return;
   __syncthreads();
   line_dft<T, E, NT, FWD>(lds, a, lg, nl);
   double acc = 0.0;
-  for (int idx = threadIdx.x; idx < total; idx += NT) {
-    const int line = idx & (nl - 1);
-    const int el = idx >> lg;
-    if (line >= nv) continue;
-    const cpx<T> val = lds[idx];
-    if (DST == DST_BUF) {
-      a.buf[base + line + el * sl] = val;
-    } else {
+  if (DST == DST_BUF) {
+    staged<NT, K, cpx<T>>(
+        total, [&](int idx) { return lds[idx]; },
+        [&](int idx, const cpx<T>& v) {
+          const int line = idx & (nl - 1), el = idx >> lg;
+          if (line < nv) a.buf[base + line + el * sl] = v;
+        });
+  } else if (DST == DST_U && interior) {
+    staged<NT, (K > 2 ? K / 2 : 1), Fetched<T>>(
+        total,
+        [&](int idx) {
+          const int line = idx & (nl - 1), el = idx >> lg;
+          Fetched<T> f;
+          f.m = lds[idx];
+          f.ops = epi_fetch_u(a.epi, base + (line < nv ? line : 0) + el * sl);
+          return f;
+        },
+        [&](int idx, const Fetched<T>& f) {
+          const int line = idx & (nl - 1), el = idx >> lg;
+          if (line >= nv) return;
+          acc += epi_commit_u(a.epi, base + line + el * sl, scale_sq2(f.m), f.ops);
+        });
+  } else if (DST == DST_U) {
+    staged<NT, (K > 2 ? K / 2 : 1), Fetched<T>>(
+        total,
+        [&](int idx) {
+          const int line = idx & (nl - 1), el = idx >> lg;
+          const int k1 = k10 + (line < nv ? line : 0);
+          const int k2 = a.axis == 1 ? el : o;
+          const int k3 = a.axis == 1 ? o : el;
+          Fetched<T> f;
+          f.m = lds[idx];
+          f.ops = epi_fetch_u(a.epi, k1 + (int64_t)a.H1 * (k2 + (int64_t)a.g.n[1] * k3));
+          return f;
+        },
+        [&](int idx, const Fetched<T>& f) {
+          const int line = idx & (nl - 1), el = idx >> lg;
+          if (line >= nv) return;
+          const int k1 = k10 + line;
+          const int k2 = a.axis == 1 ? el : o;
+          const int k3 = a.axis == 1 ? o : el;
+          const int64_t e = k1 + (int64_t)a.H1 * (k2 + (int64_t)a.g.n[1] * k3);
+          acc += epi_commit_u(a.epi, e, v_to_u<D, T>(f.m, k1, k2, k3, a.g), f.ops);
+        });
+  } else {
+    for (int idx = threadIdx.x; idx < total; idx += NT) {
+      const int line = idx & (nl - 1);
+      const int el = idx >> lg;
+      if (line >= nv) continue;
       const int k2 = a.axis == 1 ? el : o;
       const int k3 = a.axis == 1 ? o : el;
-      acc += spec_store<D, T, DST>(a, k10 + line, k2, k3, val);
+      acc += spec_store<D, T, DST>(a, k10 + line, k2, k3, lds[idx]);
     }
   }
   if (DST != DST_BUF && a.epi.dot_part) store_partial(a.epi.dot_part, acc, tile);
@@ -440,7 +618,7 @@ __device__ __forceinline__ void lines_body(const PassArgs<T>& a, int tile) {
 
 template <typename T, int NT, int E, int D, int SRC, int DST, bool FWD>
 __global__ __launch_bounds__(NT) void k_lines(PassArgs<T> a) {
-  lines_body<T, NT, E, D, SRC, DST, FWD>(a, blockIdx.x);
+  lines_body<T, NT, E, D, SRC, DST, FWD>(a, xcd_block());
 }
 
 // block -> (instance, tile) map of a launch over several instances
@@ -451,96 +629,198 @@ struct BSeg {
 
 template <typename T, int NT, int E, int D, int SRC, int DST, bool FWD>
 __global__ __launch_bounds__(NT) void kb_lines(const PassArgs<T>* __restrict__ tab, BSeg seg) {
-  const int i = uniform(seg.inst[blockIdx.x]);
+  const int b = xcd_block();
+  const int i = uniform(seg.inst[b]);
   const PassArgs<T> a = tab[i];  // loaded before any store: stays in scalar registers
-  lines_body<T, NT, E, D, SRC, DST, FWD>(a, (int)blockIdx.x - uniform(seg.first[i]));
+  lines_body<T, NT, E, D, SRC, DST, FWD>(a, b - uniform(seg.first[i]));
 }
 
 __device__ __forceinline__ bool miss_bit(const uint32_t* m, int64_t s) {
   return m && ((m[s >> 5] >> (s & 31)) & 1u);
 }
 
+// staged values of the dim-1 pass
+template <typename T>
+struct Pair {
+  cpx<T> A, B;
+  int mode;
+};
+
+template <typename T>
+struct Masked {
+  cpx<T> z;
+  uint32_t wa, wb;
+};
+
+template <typename T>
+struct Fetched2 {
+  cpx<T> X, Y;
+  EpiOps<T> ops;
+};
+
 template <typename T, int NT, int E, int D, int SRC, int DST, int MODE>
 __device__ __forceinline__ void dim1_body(const PassArgs<T>& a, int tile) {
   extern __shared__ unsigned char smem_raw[];
   cpx<T>* lds = reinterpret_cast<cpx<T>*>(smem_raw);
-  if (a.done && *a.done) return;
+  const int dflag = a.done ? *a.done : 0;   // read beside the loads
   const int N1 = a.len, H1 = a.H1;
   const int lg = a.lg, np = 1 << lg, ps = a.ps;
   const int p0 = tile << lg;
   const bool even = (N1 & 1) == 0;
+  constexpr int K = stage_k<T>();
+  const int nl1 = a.nlines - 1;
   if (MODE != MODE_R2C) {
     const int tot = H1 << lg;
-    for (int idx = threadIdx.x; idx < tot; idx += NT) {
-      const int p = udiv(idx, H1, a.h1m);
-      const int k = idx - p * H1;
-      const int la = 2 * (p0 + p), lb = la + 1;
-      cpx<T> A = {(T)0, (T)0}, B = {(T)0, (T)0};
-      if (la < a.nlines) {
-        if (SRC == SRC_BUF) {
-          A = a.buf[(int64_t)la * H1 + k];
-          if (lb < a.nlines) B = a.buf[(int64_t)lb * H1 + k];
-        } else {
-          A = spec_load<D, T, SRC>(a, k, 0, 0);   // 1D: a single line
-        }
-      }
-      if (k == 0 || (even && k == H1 - 1)) {  // DC / Nyquist of a real line are real
-        A.y = 0;
-        B.y = 0;
-      }
-      lds[k * ps + p] = {A.x - B.y, A.y + B.x};
-      if (k > 0 && N1 - k != k) lds[(N1 - k) * ps + p] = {A.x + B.y, B.x - A.y};
-    }
+    staged<NT, K, Pair<T>>(
+        tot,
+        [&](int idx) {
+          const int p = udiv(idx, H1, a.h1m);
+          const int k = idx - p * H1;
+          const int la = 2 * (p0 + p);
+          Pair<T> r;
+          r.mode = 0;
+          r.B = {(T)0, (T)0};
+          if (SRC == SRC_BUF) {
+            r.A = a.buf[(int64_t)min(la, nl1) * H1 + k];
+            r.B = a.buf[(int64_t)min(la + 1, nl1) * H1 + k];
+          } else if (SRC == SRC_U) {
+            r.A = a.u_in[u_src<D>(k, 0, 0, a.g, a.H1, r.mode)];   // 1D: a single line
+          } else {
+            r.A = spec_load<D, T, SRC>(a, k, 0, 0);
+          }
+          return r;
+        },
+        [&](int idx, const Pair<T>& r) {
+          const int p = udiv(idx, H1, a.h1m);
+          const int k = idx - p * H1;
+          const int la = 2 * (p0 + p), lb = la + 1;
+          const cpx<T> zero = {(T)0, (T)0};
+          cpx<T> A = la < a.nlines ? (SRC == SRC_U ? u_map(r.A, r.mode) : r.A) : zero;
+          cpx<T> B = (SRC == SRC_BUF && lb < a.nlines) ? r.B : zero;
+          if (k == 0 || (even && k == H1 - 1)) {  // DC / Nyquist of a real line are real
+            A.y = 0;
+            B.y = 0;
+          }
+          lds[k * ps + p] = {A.x - B.y, A.y + B.x};
+          if (k > 0 && N1 - k != k) lds[(N1 - k) * ps + p] = {A.x + B.y, B.x - A.y};
+        });
+    if (dflag) return;
     __syncthreads();
     line_dft<T, E, NT, false>(lds, a, lg, ps);
     const int totr = N1 << lg;
-    for (int idx = threadIdx.x; idx < totr; idx += NT) {
-      const int p = idx & (np - 1);
-      const int n = idx >> lg;
-      const int la = 2 * (p0 + p), lb = la + 1;
-      cpx<T> z = lds[n * ps + p];
-      const T sa = (la < a.nlines && !miss_bit(a.miss, (int64_t)la * N1 + n)) ? a.scale : (T)0;
-      const T sb = (lb < a.nlines && !miss_bit(a.miss, (int64_t)lb * N1 + n)) ? a.scale : (T)0;
-      z.x *= sa;
-      z.y *= sb;
-      if (MODE == MODE_C2R) {
-        if (la < a.nlines) a.real_out[(int64_t)la * N1 + n] = z.x;
-        if (lb < a.nlines) a.real_out[(int64_t)lb * N1 + n] = z.y;
-      } else {
-        lds[n * ps + p] = z;
-      }
-    }
+    staged<NT, K, Masked<T>>(
+        totr,
+        [&](int idx) {
+          const int p = idx & (np - 1);
+          const int n = idx >> lg;
+          const int la = 2 * (p0 + p);
+          Masked<T> r;
+          r.z = lds[n * ps + p];
+          r.wa = r.wb = 0u;
+          if (a.miss) {
+            r.wa = a.miss[((int64_t)min(la, nl1) * N1 + n) >> 5];
+            r.wb = a.miss[((int64_t)min(la + 1, nl1) * N1 + n) >> 5];
+          }
+          return r;
+        },
+        [&](int idx, const Masked<T>& r) {
+          const int p = idx & (np - 1);
+          const int n = idx >> lg;
+          const int la = 2 * (p0 + p), lb = la + 1;
+          const int64_t sa_i = (int64_t)la * N1 + n, sb_i = (int64_t)lb * N1 + n;
+          const bool ma = (r.wa >> (sa_i & 31)) & 1u;
+          const bool mb = (r.wb >> (sb_i & 31)) & 1u;
+          const T sa = (la < a.nlines && !ma) ? a.scale : (T)0;
+          const T sb = (lb < a.nlines && !mb) ? a.scale : (T)0;
+          cpx<T> z = r.z;
+          z.x *= sa;
+          z.y *= sb;
+          if (MODE == MODE_C2R) {
+            if (la < a.nlines) a.real_out[sa_i] = z.x;
+            if (lb < a.nlines) a.real_out[sb_i] = z.y;
+          } else {
+            lds[n * ps + p] = z;
+          }
+        });
     if (MODE == MODE_C2R) return;
     __syncthreads();
   } else {
     const int totr = N1 << lg;
-    for (int idx = threadIdx.x; idx < totr; idx += NT) {
-      const int p = udiv(idx, N1, a.n1m);
-      const int n = idx - p * N1;
-      const int la = 2 * (p0 + p), lb = la + 1;
-      const T xa = la < a.nlines ? a.real_in[(int64_t)la * N1 + n] * a.scale : (T)0;
-      const T xb = lb < a.nlines ? a.real_in[(int64_t)lb * N1 + n] * a.scale : (T)0;
-      lds[n * ps + p] = {xa, xb};
-    }
+    staged<NT, K, cpx<T>>(
+        totr,
+        [&](int idx) {
+          const int p = udiv(idx, N1, a.n1m);
+          const int n = idx - p * N1;
+          const int la = 2 * (p0 + p);
+          return cpx<T>{a.real_in[(int64_t)min(la, nl1) * N1 + n],
+                        a.real_in[(int64_t)min(la + 1, nl1) * N1 + n]};
+        },
+        [&](int idx, const cpx<T>& r) {
+          const int p = udiv(idx, N1, a.n1m);
+          const int n = idx - p * N1;
+          const int la = 2 * (p0 + p), lb = la + 1;
+          const T xa = la < a.nlines ? r.x * a.scale : (T)0;
+          const T xb = lb < a.nlines ? r.y * a.scale : (T)0;
+          lds[n * ps + p] = {xa, xb};
+        });
+    if (dflag) return;
     __syncthreads();
   }
   line_dft<T, E, NT, true>(lds, a, lg, ps);
   double acc = 0.0;
   const int tot = H1 << lg;
   const T half = (T)0.5;
-  for (int idx = threadIdx.x; idx < tot; idx += NT) {
-    const int p = udiv(idx, H1, a.h1m);
-    const int k = idx - p * H1;
-    const int la = 2 * (p0 + p), lb = la + 1;
-    if (la >= a.nlines) continue;
-    const cpx<T> X = lds[k * ps + p];
-    const cpx<T> Y = lds[(k == 0 ? 0 : N1 - k) * ps + p];
-    const cpx<T> Va = {half * (X.x + Y.x), half * (X.y - Y.y)};
-    const cpx<T> Vb = {half * (X.y + Y.y), half * (Y.x - X.x)};
-    if (DST == DST_BUF) {
-      a.buf[(int64_t)la * H1 + k] = Va;
-      if (lb < a.nlines) a.buf[(int64_t)lb * H1 + k] = Vb;
-    } else {
+  if (DST == DST_BUF) {
+    staged<NT, K, Pair<T>>(
+        tot,
+        [&](int idx) {
+          const int p = udiv(idx, H1, a.h1m);
+          const int k = idx - p * H1;
+          Pair<T> r;
+          r.A = lds[k * ps + p];
+          r.B = lds[(k == 0 ? 0 : N1 - k) * ps + p];
+          r.mode = 0;
+          return r;
+        },
+        [&](int idx, const Pair<T>& r) {
+          const int p = udiv(idx, H1, a.h1m);
+          const int k = idx - p * H1;
+          const int la = 2 * (p0 + p), lb = la + 1;
+          if (la >= a.nlines) return;
+          const cpx<T> X = r.A, Y = r.B;
+          const cpx<T> Va = {half * (X.x + Y.x), half * (X.y - Y.y)};
+          const cpx<T> Vb = {half * (X.y + Y.y), half * (Y.x - X.x)};
+          a.buf[(int64_t)la * H1 + k] = Va;
+          if (lb < a.nlines) a.buf[(int64_t)lb * H1 + k] = Vb;
+        });
+  } else if (DST == DST_U) {   // 1D: line 0 only
+    staged<NT, (K > 2 ? K / 2 : 1), Fetched2<T>>(
+        tot,
+        [&](int idx) {
+          const int p = udiv(idx, H1, a.h1m);
+          const int k = idx - p * H1;
+          Fetched2<T> f;
+          f.X = lds[k * ps + p];
+          f.Y = lds[(k == 0 ? 0 : N1 - k) * ps + p];
+          f.ops = epi_fetch_u(a.epi, k);
+          return f;
+        },
+        [&](int idx, const Fetched2<T>& f) {
+          const int p = udiv(idx, H1, a.h1m);
+          const int k = idx - p * H1;
+          if (2 * (p0 + p) >= a.nlines) return;
+          const cpx<T> Va = {half * (f.X.x + f.Y.x), half * (f.X.y - f.Y.y)};
+          acc += epi_commit_u(a.epi, k, v_to_u<D, T>(Va, k, 0, 0, a.g), f.ops);
+        });
+  } else {
+    for (int idx = threadIdx.x; idx < tot; idx += NT) {
+      const int p = udiv(idx, H1, a.h1m);
+      const int k = idx - p * H1;
+      const int la = 2 * (p0 + p);
+      if (la >= a.nlines) continue;
+      const cpx<T> X = lds[k * ps + p];
+      const cpx<T> Y = lds[(k == 0 ? 0 : N1 - k) * ps + p];
+      const cpx<T> Va = {half * (X.x + Y.x), half * (X.y - Y.y)};
       acc += spec_store<D, T, DST>(a, k, 0, 0, Va);   // 1D: line 0 only
     }
   }
@@ -549,14 +829,15 @@ __device__ __forceinline__ void dim1_body(const PassArgs<T>& a, int tile) {
 
 template <typename T, int NT, int E, int D, int SRC, int DST, int MODE>
 __global__ __launch_bounds__(NT) void k_dim1(PassArgs<T> a) {
-  dim1_body<T, NT, E, D, SRC, DST, MODE>(a, blockIdx.x);
+  dim1_body<T, NT, E, D, SRC, DST, MODE>(a, xcd_block());
 }
 
 template <typename T, int NT, int E, int D, int SRC, int DST, int MODE>
 __global__ __launch_bounds__(NT) void kb_dim1(const PassArgs<T>* __restrict__ tab, BSeg seg) {
-  const int i = uniform(seg.inst[blockIdx.x]);
+  const int b = xcd_block();
+  const int i = uniform(seg.inst[b]);
   const PassArgs<T> a = tab[i];  // loaded before any store: stays in scalar registers
-  dim1_body<T, NT, E, D, SRC, DST, MODE>(a, (int)blockIdx.x - uniform(seg.first[i]));
+  dim1_body<T, NT, E, D, SRC, DST, MODE>(a, b - uniform(seg.first[i]));
 }
 
 // ===========================================================================
@@ -735,23 +1016,25 @@ __device__ __forceinline__ void row_body(const PassArgs<T>& a, int tile) {
 
 template <typename T, int NT, int E, int SRC, int DST, int MODE>
 __global__ __launch_bounds__(NT) void k_col(PassArgs<T> a) {
-  col_body<T, NT, E, SRC, DST, MODE>(a, blockIdx.x);
+  col_body<T, NT, E, SRC, DST, MODE>(a, xcd_block());
 }
 template <typename T, int NT, int E, int SRC, int DST, int MODE>
 __global__ __launch_bounds__(NT) void kb_col(const PassArgs<T>* __restrict__ tab, BSeg seg) {
-  const int i = uniform(seg.inst[blockIdx.x]);
+  const int b = xcd_block();
+  const int i = uniform(seg.inst[b]);
   const PassArgs<T> a = tab[i];  // loaded before any store: stays in scalar registers
-  col_body<T, NT, E, SRC, DST, MODE>(a, (int)blockIdx.x - uniform(seg.first[i]));
+  col_body<T, NT, E, SRC, DST, MODE>(a, b - uniform(seg.first[i]));
 }
 template <typename T, int NT, int E, int MODE>
 __global__ __launch_bounds__(NT) void k_row(PassArgs<T> a) {
-  row_body<T, NT, E, MODE>(a, blockIdx.x);
+  row_body<T, NT, E, MODE>(a, xcd_block());
 }
 template <typename T, int NT, int E, int MODE>
 __global__ __launch_bounds__(NT) void kb_row(const PassArgs<T>* __restrict__ tab, BSeg seg) {
-  const int i = uniform(seg.inst[blockIdx.x]);
+  const int b = xcd_block();
+  const int i = uniform(seg.inst[b]);
   const PassArgs<T> a = tab[i];  // loaded before any store: stays in scalar registers
-  row_body<T, NT, E, MODE>(a, (int)blockIdx.x - uniform(seg.first[i]));
+  row_body<T, NT, E, MODE>(a, b - uniform(seg.first[i]));
 }
 
 // beta <-> u (exact: a signed permutation), one thread per half-spectrum element.
