@@ -33,6 +33,8 @@ SIGNATURES = {
     "csipm_dft_to_beta": (_i, [_vp, _vp, _vp, _i]),
     "csipm_m_perp_beta": (_i, [_vp, _vp, _vp]),
     "csipm_m_perpt_z": (_i, [_vp, _vp, _vp]),
+    "csipm_reconstruct": (_i, [_vp, _vp, _vp]),
+    "csipm_mask_index_missing": (_i, [_vp, _i64, _vp, ctypes.POINTER(_i64), _vp]),
     "csipm_mtm": (_i, [_vp, _vp, _vp]),
     "csipm_kkt_apply": (_i, [_vp, _vp, _vp, _vp, _vp, _d, _d]),
     "csipm_precond_build": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),

@@ -1185,6 +1185,107 @@ int csipm_m_perp_beta(csipm_op_t h, const void* beta, void* y) {
   return apply_m_perp_beta<float, 3>(h, (const float*)beta, (float*)y);
 }
 
+namespace csipm {
+
+// ---------------------------------------------------------------------------
+// index_missing from a sampling mask / weights array (zero = not observed):
+// the 0-based linear indices of its zero entries in increasing order, by a
+// count pass, an exclusive scan of the per-block counts and an ordered write
+// (wave ballots for the order inside a chunk of 256 entries).
+// ---------------------------------------------------------------------------
+constexpr int kMaskChunks = 16;   // chunks of 256 entries per block
+
+__global__ __launch_bounds__(256) void k_mask_count(const double* __restrict__ w, int64_t n,
+                                                    int64_t* __restrict__ counts) {
+  const int64_t base = (int64_t)blockIdx.x * 256 * kMaskChunks;
+  double c = 0.0;
+#pragma unroll
+  for (int j = 0; j < kMaskChunks; ++j) {
+    const int64_t e = base + j * 256 + threadIdx.x;
+    if (e < n && w[e] == 0.0) c += 1.0;
+  }
+  const double s = block_reduce(c, OpSum());
+  if (threadIdx.x == 0) counts[blockIdx.x] = (int64_t)s;
+}
+
+__global__ __launch_bounds__(256) void k_mask_write(const double* __restrict__ w, int64_t n,
+                                                    const int64_t* __restrict__ offsets,
+                                                    int64_t* __restrict__ out) {
+  __shared__ int wave_tot[4];
+  const int64_t base = (int64_t)blockIdx.x * 256 * kMaskChunks;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int64_t run = offsets[blockIdx.x];
+  for (int j = 0; j < kMaskChunks; ++j) {
+    const int64_t e = base + j * 256 + threadIdx.x;
+    const bool z = e < n && w[e] == 0.0;
+    const uint64_t bal = __ballot(z);
+    const int below = __popcll(bal & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))));
+    if (lane == 0) wave_tot[wave] = __popcll(bal);
+    __syncthreads();
+    int woff = 0, tot = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (q < wave) woff += wave_tot[q];
+      tot += wave_tot[q];
+    }
+    if (z) out[run + woff + below] = e;
+    run += tot;
+    __syncthreads();
+  }
+}
+
+}  // namespace csipm
+
+int csipm_reconstruct(csipm_op_t h, const void* beta, void* x) {
+  if (!h || !beta || !x) return fail(CSIPM_ERR_ARG, "null argument");
+  // M_perp_beta of the same handle with no sample missing: the pass
+  // arguments are taken at launch, so the mask is restored right after
+  uint32_t* miss = h->miss;
+  h->miss = nullptr;
+  const int rc = csipm_m_perp_beta(h, beta, x);
+  h->miss = miss;
+  return rc;
+}
+
+int csipm_mask_index_missing(const double* weights, int64_t n, int64_t* index_out,
+                             int64_t* count, void* stream) {
+  if (!count || n < 0 || (n > 0 && (!weights || !index_out)))
+    return fail(CSIPM_ERR_ARG, "null argument or negative size");
+  *count = 0;
+  if (n == 0) return CSIPM_OK;
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t per = 256 * (int64_t)kMaskChunks;
+  const int64_t nb = (n + per - 1) / per;
+  if (nb > INT32_MAX) return fail(CSIPM_ERR_ARG, "mask too large");
+  int64_t* d = nullptr;
+  HIPCHK(hipMalloc((void**)&d, sizeof(int64_t) * (size_t)nb));
+  hipLaunchKernelGGL(k_mask_count, dim3((unsigned)nb), dim3(256), 0, st, weights, n, d);
+  hipError_t e = hipGetLastError();
+  std::vector<int64_t> c((size_t)nb);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(c.data(), d, sizeof(int64_t) * (size_t)nb, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  int64_t tot = 0;
+  for (auto& v : c) {
+    const int64_t t = v;
+    v = tot;
+    tot += t;
+  }
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(d, c.data(), sizeof(int64_t) * (size_t)nb, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_mask_write, dim3((unsigned)nb), dim3(256), 0, st, weights, n, d,
+                       index_out);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  hipFree(d);
+  if (e != hipSuccess)
+    return fail(CSIPM_ERR_HIP, "csipm_mask_index_missing: %s", hipGetErrorString(e));
+  *count = tot;
+  return CSIPM_OK;
+}
+
 int csipm_m_perpt_z(csipm_op_t h, const void* z, void* beta) {
   if (!h || !beta || !z) return fail(CSIPM_ERR_ARG, "null argument");
   if (OP_DOUBLE(h)) return dispatch_m_perpt_z<double, SP_BETA>(h, (const double*)z, (double*)beta);

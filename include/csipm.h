@@ -69,6 +69,17 @@ int csipm_op_create(csipm_op_t* h, int dim, const int64_t sizes[3], int rdft,
                     int precision, const int64_t* missing, int64_t n_missing,
                     void* stream);
 int csipm_op_destroy(csipm_op_t h);
+/* index_missing of a sampling mask / weights array: the 0-based linear
+ * (column-major) indices of its zero entries, in increasing order, as the
+ * loops of benchmarks/mastodonte.jl:38-50 build them (1-based there).
+ *   weights    device, n doubles (the mask; nonzero = observed)
+ *   index_out  device, room for n int64 (receives *count entries)
+ *   count      host: number of missing samples
+ * Synchronous on stream.  The result feeds csipm_op_create's `missing`
+ * after a copy to the host.                                               */
+int csipm_mask_index_missing(const double* weights, int64_t n, int64_t* index_out,
+                             int64_t* count, void* stream);
+
 /* Change the stream later calls on h are issued to. */
 int csipm_op_set_stream(csipm_op_t h, void* stream);
 /* Query N (length of beta) and Nc (complex entries of the half spectrum). */
@@ -85,6 +96,11 @@ int csipm_dft_to_beta(csipm_op_t h, const double* v, double* beta, int full);
 /* y = M beta: unitary inverse DFT of beta_to_DFT(beta), zeroed at missing
  * samples.  M_perp_beta (src/fft_utils.jl:95-117).  T* in/out, N each.   */
 int csipm_m_perp_beta(csipm_op_t h, const void* beta, void* y);
+/* Recovered signal x = sqrt(N) * real(ifft(beta_to_DFT(beta))): M_perp_beta
+ * of this handle with no sample missing, the reconstruction step the
+ * reference's real-data driver performs after a solve
+ * (benchmarks/mastodonte.jl:109-116).  T* in/out, N each.               */
+int csipm_reconstruct(csipm_op_t h, const void* beta, void* x);
 /* beta = M^T z = DFT_to_beta(R2C(z)/sqrt(N)).  M_perpt_z (src/fft_utils.jl:73-93). */
 int csipm_m_perpt_z(csipm_op_t h, const void* z, void* beta);
 /* y = M^T M x.  M_perpt_M_perp_vec (src/fft_utils.jl:119-123).          */
