@@ -34,6 +34,19 @@ SIGNATURES = {
     "csipm_m_perp_beta": (_i, [_vp, _vp, _vp]),
     "csipm_m_perpt_z": (_i, [_vp, _vp, _vp]),
     "csipm_reconstruct": (_i, [_vp, _vp, _vp]),
+    "csipm_gondzio_obj": (_i, [_vp, _vp, _d, ctypes.POINTER(_d)]),
+    "csipm_gondzio_grad": (_i, [_vp, _vp, _d, _vp]),
+    "csipm_gondzio_cons": (_i, [_vp, _vp, _vp, _vp]),
+    "csipm_gondzio_jtprod": (_i, [_vp, _vp, _vp]),
+    "csipm_gondzio_kkt_apply": (_i, [_vp, _vp, _vp, _vp, _vp, _d, _d]),
+    "csipm_gondzio_kkt_mul_full": (_i, [_vp, _vp, _vp, _d, _d]),
+    "csipm_gondzio_precond_build": (_i, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "csipm_gondzio_condense_rhs": (_i, [_vp, _vp, _vp]),
+    "csipm_gondzio_unpack_solution": (_i, [_vp, _vp, _vp]),
+    "csipm_gondzio_cg_solve": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _d, _i64,
+                                    ctypes.POINTER(_i64), ctypes.POINTER(_d),
+                                    ctypes.POINTER(_i)]),
+    "csipm_gondzio_solve": (_i, [_vp, _vp, _d, _d, _i64, _d, _i64, _vp, _vp, _vp]),
     "csipm_mask_index_missing": (_i, [_vp, _i64, _vp, ctypes.POINTER(_i64), _vp]),
     "csipm_mtm": (_i, [_vp, _vp, _vp]),
     "csipm_kkt_apply": (_i, [_vp, _vp, _vp, _vp, _vp, _d, _d]),

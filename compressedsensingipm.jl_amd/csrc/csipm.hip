@@ -125,6 +125,7 @@ struct csipm_op {
   CgState* cg = nullptr;         // device
   CgState* cg_host = nullptr;    // pinned mirror
   void* scratch = nullptr;       // N-vector scratch of the NLP entry points (op precision)
+  void* gz = nullptr;            // 2N-vector scratch of the Gondzio entry points
   Engine* solo = nullptr;        // batch-of-one engine of csipm_solve / csipm_cg_solve
   bool has64() const { return prec == CSIPM_F64 || prec == CSIPM_MIXED; }
   bool has32() const { return prec == CSIPM_F32 || prec == CSIPM_MIXED; }
@@ -898,12 +899,14 @@ int admm_solo(csipm_op* h, const double* z0, double lam, double rho, int64_t max
 // csipm_solve: the interior-point driver on a batch of one
 template <typename TC>
 int solve_solo(csipm_op* h, const double* z0, double lam, double tol, int64_t max_iter,
-               double cg_rtol, int64_t cg_itmax, double* beta_out, double* info) {
+               double cg_rtol, int64_t cg_itmax, double* beta_out, double* info,
+               double* c_out = nullptr) {
   const auto t0 = std::chrono::steady_clock::now();
   Engine* E = nullptr;
   CHK(solo_engine(h, &E));
   int id = -1;
   CHK(engine_admit(E, h, z0, lam, beta_out, cg_itmax, &id));
+  E->inst[(size_t)id].c_out = c_out;
   const int mi = (int)std::min<int64_t>(max_iter, 1 << 30);
   int rc = engine_run<TC>(E, tol, mi, cg_rtol);
   if (rc) {
@@ -920,6 +923,254 @@ int solve_solo(csipm_op* h, const double* z0, double lam, double tol, int64_t ma
 // ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
+// ---------------------------------------------------------------------------
+// Gondzio formulation (src/gondzio_model.jl, src/gondzio_kkt.jl):
+// x = [p; q; r], beta = p - q, c(x) = z0 - M(p - q) - r.  The operator work
+// is the handle's own M / M^T / M^T M; the vector work is elementwise.
+// ---------------------------------------------------------------------------
+extern "C++" {
+
+template <typename F>
+__global__ __launch_bounds__(256) void k_each(int64_t n, F f) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    f(i);
+}
+
+template <typename F>
+static int each(csipm_op* h, int64_t n, F f) {
+  hipLaunchKernelGGL((k_each<F>), dim3(grid_for(n, 4)), dim3(kNT), 0, h->stream, n, f);
+  HIPCHK(hipGetLastError());
+  return CSIPM_OK;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_gz_obj_parts(const T* __restrict__ x, int64_t n,
+                                                      double* part) {
+  double a[2] = {0, 0};
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double r = x[2 * n + i];
+    a[0] += r * r;
+    a[1] += (double)x[i] + (double)x[n + i];
+  }
+  const int ops[2] = {0, 0};
+  store_partials<2>(part, a, ops);
+}
+
+template <typename T>
+static int gz_scratch(csipm_op* h, T** a, T** b) {
+  if (!h->gz) HIPCHK(hipMalloc(&h->gz, 2 * sizeof(T) * (size_t)h->N));
+  *a = (T*)h->gz;
+  *b = (T*)h->gz + h->N;
+  return CSIPM_OK;
+}
+
+template <typename T>
+static int op_m(csipm_op* h, const T* v, T* y) {
+  if (h->dim == 1) return apply_m_perp_beta<T, 1>(h, v, y);
+  if (h->dim == 2) return apply_m_perp_beta<T, 2>(h, v, y);
+  return apply_m_perp_beta<T, 3>(h, v, y);
+}
+
+template <typename T>
+static int op_mt(csipm_op* h, const T* z, T* beta) {
+  return dispatch_m_perpt_z<T, SP_BETA>(h, z, beta);
+}
+
+template <typename T>
+static int gz_obj_t(csipm_op* h, const T* x, double lam, double* f) {
+  const int gv = grid_for(h->N, 4);
+  hipLaunchKernelGGL((k_gz_obj_parts<T>), dim3(gv), dim3(kNT), 0, h->stream, x, h->N, h->part);
+  HIPCHK(hipGetLastError());
+  std::vector<double> hp(2 * (size_t)gv);
+  HIPCHK(hipMemcpyAsync(hp.data(), h->part, sizeof(double) * hp.size(), hipMemcpyDeviceToHost,
+                        h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  double rr = 0, sx = 0;
+  for (int i = 0; i < gv; ++i) {
+    rr += hp[(size_t)i];
+    sx += hp[(size_t)gv + i];
+  }
+  *f = 0.5 * rr + lam * sx;
+  return CSIPM_OK;
+}
+
+template <typename T>
+static int gz_grad_t(csipm_op* h, const T* x, double lam, T* g) {
+  const int64_t n = h->N;
+  const T l = (T)lam;
+  return each(h, n, [=] __device__(int64_t i) {
+    g[i] = l;
+    g[n + i] = l;
+    g[2 * n + i] = x[2 * n + i];
+  });
+}
+
+template <typename T>
+static int gz_cons_t(csipm_op* h, const T* x, const T* z0, T* c) {
+  const int64_t n = h->N;
+  T *d, *md;
+  CHK(gz_scratch<T>(h, &d, &md));
+  CHK(each(h, n, [=] __device__(int64_t i) { d[i] = x[i] - x[n + i]; }));
+  CHK(op_m<T>(h, d, md));
+  return each(h, n, [=] __device__(int64_t i) { c[i] = z0[i] - md[i] - x[2 * n + i]; });
+}
+
+template <typename T>
+static int gz_jtprod_t(csipm_op* h, const T* v, T* out) {
+  const int64_t n = h->N;
+  T *t, *unused;
+  CHK(gz_scratch<T>(h, &t, &unused));
+  CHK(op_mt<T>(h, v, t));
+  return each(h, n, [=] __device__(int64_t i) {
+    out[i] = -t[i];
+    out[n + i] = t[i];
+    out[2 * n + i] = -v[i];
+  });
+}
+
+template <typename T>
+static int gz_kkt_apply_t(csipm_op* h, const T* x, T* y, const T* Wp, const T* Wq, double alpha,
+                          double beta) {
+  const int64_t n = h->N;
+  T *d, *m;
+  CHK(gz_scratch<T>(h, &d, &m));
+  CHK(each(h, n, [=] __device__(int64_t i) { d[i] = x[i] - x[n + i]; }));
+  CHK((dispatch_mtm<T, SP_BETA>(h, d, epi_store<T>(m), nullptr)));
+  const T a = (T)alpha, b = (T)beta;
+  return each(h, n, [=] __device__(int64_t i) {
+    T yp = a * (Wp[i] * x[i] + m[i]);
+    T yq = a * (Wq[i] * x[n + i] - m[i]);
+    if (b != (T)0) {
+      yp += b * y[i];
+      yq += b * y[n + i];
+    }
+    y[i] = yp;
+    y[n + i] = yq;
+  });
+}
+
+template <typename T>
+static int gz_kkt_full_t(csipm_op* h, const T* x, T* y, double alpha, double beta) {
+  const int64_t n = h->N;
+  T *t, *s;
+  CHK(gz_scratch<T>(h, &t, &s));
+  CHK(op_mt<T>(h, x + 3 * n, t));
+  // s = M (x_q - x_p), formed in y's r block's place only after t is taken
+  T* dqp = s;
+  CHK(each(h, n, [=] __device__(int64_t i) { dqp[i] = x[n + i] - x[i]; }));
+  T* ms = nullptr;
+  HIPCHK(hipMallocAsync((void**)&ms, sizeof(T) * (size_t)n, h->stream));
+  int rc = op_m<T>(h, dqp, ms);
+  const T a = (T)alpha, b = (T)beta;
+  if (!rc)
+    rc = each(h, n, [=] __device__(int64_t i) {
+      const T xr = x[2 * n + i], xy = x[3 * n + i];
+      T v0 = -a * t[i], v1 = a * t[i], v2 = a * (xr - xy), v3 = a * (ms[i] - xr);
+      if (b != (T)0) {
+        v0 += b * y[i];
+        v1 += b * y[n + i];
+        v2 += b * y[2 * n + i];
+        v3 += b * y[3 * n + i];
+      }
+      y[i] = v0;
+      y[n + i] = v1;
+      y[2 * n + i] = v2;
+      y[3 * n + i] = v3;
+    });
+  (void)hipFreeAsync(ms, h->stream);
+  return rc;
+}
+
+// P = [[1 + Wp, -1], [-1, 1 + Wq]]^-1 in the closed form P11 = (1 + Wq)/det,
+// P12 = 1/det, P22 = (1 + Wp)/det, det = Wp + Wq + Wp Wq: the matrix
+// build_kkt! forms as inv = 1/(1 + Wp), S = 1/(1 + Wq - inv), P12 = inv S,
+// P11 = inv + inv S inv, P22 = S, without the difference 1 + Wq - inv
+template <typename T>
+static int gz_precond_build_t(csipm_op* h, const T* Wp, const T* Wq, T* P11, T* P12, T* P22) {
+  return each(h, h->N, [=] __device__(int64_t i) {
+    const double wp = Wp[i], wq = Wq[i];
+    const double det = wp + wq + wp * wq;
+    P11[i] = (T)((1.0 + wq) / det);
+    P12[i] = (T)(1.0 / det);
+    P22[i] = (T)((1.0 + wp) / det);
+  });
+}
+
+template <typename T>
+static int gz_condense_t(csipm_op* h, const T* w, T* rhs) {
+  const int64_t n = h->N;
+  T *sum, *t;
+  CHK(gz_scratch<T>(h, &sum, &t));
+  CHK(each(h, n, [=] __device__(int64_t i) { sum[i] = w[2 * n + i] + w[3 * n + i]; }));
+  CHK(op_mt<T>(h, sum, t));
+  return each(h, n, [=] __device__(int64_t i) {
+    rhs[i] = w[i] - t[i];
+    rhs[n + i] = w[n + i] + t[i];
+  });
+}
+
+template <typename T>
+static int gz_unpack_t(csipm_op* h, T* w, const T* dx) {
+  const int64_t n = h->N;
+  T *d, *s;
+  CHK(gz_scratch<T>(h, &d, &s));
+  CHK(each(h, n, [=] __device__(int64_t i) { d[i] = dx[i] - dx[n + i]; }));
+  CHK(op_m<T>(h, d, s));
+  return each(h, n, [=] __device__(int64_t i) {
+    const T wr = w[2 * n + i];
+    const T dr = -w[3 * n + i] - s[i];
+    w[i] = dx[i];
+    w[n + i] = dx[n + i];
+    w[2 * n + i] = dr;
+    w[3 * n + i] = dr - wr;
+  });
+}
+
+// PCG on the condensed Gondzio K by the congruence x = A y, A = 1/2 [[I, I],
+// [-I, I]]: A^T K A is the condensed FFT system with L1 = (Wp + Wq)/4,
+// L2 = (Wp - Wq)/4, right-hand side A^T b, preconditioner A^-1 P A^-T =
+// [[P11 - 2 P12 + P22, P11 - P22], [P11 - P22, P11 + 2 P12 + P22]].  PCG is
+// invariant under the change of variables (same iterates x_k = A y_k, same
+// preconditioned residual norms, so the same stopping point).
+template <typename T>
+static int gz_cg_t(csipm_op* h, const T* b, T* x, const T* Wp, const T* Wq, const T* P11,
+                   const T* P12, const T* P22, double atol, double rtol, int64_t itmax,
+                   int64_t* iters, double* resid, int* status) {
+  const int64_t n = h->N;
+  T* buf = nullptr;
+  HIPCHK(hipMalloc((void**)&buf, 9 * sizeof(T) * (size_t)n));
+  T *bt = buf, *yt = buf + 2 * n, *L1 = buf + 4 * n, *L2 = buf + 5 * n, *Q11 = buf + 6 * n,
+    *Q12 = buf + 7 * n, *Q22 = buf + 8 * n;
+  int rc = each(h, n, [=] __device__(int64_t i) {
+    bt[i] = (T)0.5 * (b[i] - b[n + i]);
+    bt[n + i] = (T)0.5 * (b[i] + b[n + i]);
+    L1[i] = (Wp[i] + Wq[i]) * (T)0.25;
+    L2[i] = (Wp[i] - Wq[i]) * (T)0.25;
+    Q11[i] = P11[i] - (T)2 * P12[i] + P22[i];
+    Q12[i] = P11[i] - P22[i];
+    Q22[i] = P11[i] + (T)2 * P12[i] + P22[i];
+  });
+  if (!rc)
+    rc = cg_solve_solo<T>(h, bt, yt, L1, L2, Q11, Q12, Q22, atol, rtol, itmax, iters, resid,
+                          status);
+  if (!rc)
+    rc = each(h, n, [=] __device__(int64_t i) {
+      const T yb = yt[i], yc = yt[n + i];
+      x[i] = (T)0.5 * (yb + yc);
+      x[n + i] = (T)0.5 * (yc - yb);
+    });
+  if (!rc) {
+    hipError_t e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) rc = fail(CSIPM_ERR_HIP, "gondzio cg: %s", hipGetErrorString(e));
+  }
+  (void)hipFree(buf);
+  return rc;
+}
+
+}  // extern "C++"
+
 extern "C" {
 
 const char* csipm_version(void) { return CSIPM_VERSION; }
@@ -934,7 +1185,7 @@ int csipm_op_destroy(csipm_op_t h) {
       for (void* b : {ps->chirp[i], ps->bfw[i], ps->binv[i]})
         if (b) (void)hipFree(b);
   delete h->solo;
-  void* bufs[] = {h->miss, h->valid, h->buf, h->part, h->scratch, h->dplan};
+  void* bufs[] = {h->miss, h->valid, h->buf, h->part, h->scratch, h->gz, h->dplan};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   delete h;
@@ -1697,6 +1948,126 @@ const char* csipm_batch_launch_name(csipm_batch_t b, int k) {
 int csipm_batch_stats(csipm_batch_t b, double* out) {
   if (!b || !out) return fail(CSIPM_ERR_ARG, "null argument");
   for (int i = 0; i < Engine::ST_N; ++i) out[i] = b->E->stats[i];
+  return CSIPM_OK;
+}
+
+
+int csipm_gondzio_obj(csipm_op_t h, const void* x, double lambda, double* f) {
+  if (!h || !x || !f) return fail(CSIPM_ERR_ARG, "null argument");
+  return DISPATCH_T(h, gz_obj_t<double>(h, (const double*)x, lambda, f),
+                    gz_obj_t<float>(h, (const float*)x, lambda, f));
+}
+
+int csipm_gondzio_grad(csipm_op_t h, const void* x, double lambda, void* g) {
+  if (!h || !x || !g) return fail(CSIPM_ERR_ARG, "null argument");
+  return DISPATCH_T(h, gz_grad_t<double>(h, (const double*)x, lambda, (double*)g),
+                    gz_grad_t<float>(h, (const float*)x, lambda, (float*)g));
+}
+
+int csipm_gondzio_cons(csipm_op_t h, const void* x, const void* z0, void* c) {
+  if (!h || !x || !z0 || !c) return fail(CSIPM_ERR_ARG, "null argument");
+  return DISPATCH_T(h, gz_cons_t<double>(h, (const double*)x, (const double*)z0, (double*)c),
+                    gz_cons_t<float>(h, (const float*)x, (const float*)z0, (float*)c));
+}
+
+int csipm_gondzio_jtprod(csipm_op_t h, const void* v, void* out) {
+  if (!h || !v || !out) return fail(CSIPM_ERR_ARG, "null argument");
+  return DISPATCH_T(h, gz_jtprod_t<double>(h, (const double*)v, (double*)out),
+                    gz_jtprod_t<float>(h, (const float*)v, (float*)out));
+}
+
+int csipm_gondzio_kkt_apply(csipm_op_t h, const void* x, void* y, const void* Wp, const void* Wq,
+                            double alpha, double beta) {
+  if (!h || !x || !y || !Wp || !Wq) return fail(CSIPM_ERR_ARG, "null argument");
+  return DISPATCH_T(h,
+                    gz_kkt_apply_t<double>(h, (const double*)x, (double*)y, (const double*)Wp,
+                                           (const double*)Wq, alpha, beta),
+                    gz_kkt_apply_t<float>(h, (const float*)x, (float*)y, (const float*)Wp,
+                                          (const float*)Wq, alpha, beta));
+}
+
+int csipm_gondzio_kkt_mul_full(csipm_op_t h, const void* x, void* y, double alpha, double beta) {
+  if (!h || !x || !y) return fail(CSIPM_ERR_ARG, "null argument");
+  return DISPATCH_T(h, gz_kkt_full_t<double>(h, (const double*)x, (double*)y, alpha, beta),
+                    gz_kkt_full_t<float>(h, (const float*)x, (float*)y, alpha, beta));
+}
+
+int csipm_gondzio_precond_build(csipm_op_t h, const void* Wp, const void* Wq, void* P11,
+                                void* P12, void* P22) {
+  if (!h || !Wp || !Wq || !P11 || !P12 || !P22) return fail(CSIPM_ERR_ARG, "null argument");
+  return DISPATCH_T(h,
+                    gz_precond_build_t<double>(h, (const double*)Wp, (const double*)Wq,
+                                               (double*)P11, (double*)P12, (double*)P22),
+                    gz_precond_build_t<float>(h, (const float*)Wp, (const float*)Wq,
+                                              (float*)P11, (float*)P12, (float*)P22));
+}
+
+int csipm_gondzio_condense_rhs(csipm_op_t h, const void* w, void* rhs) {
+  if (!h || !w || !rhs) return fail(CSIPM_ERR_ARG, "null argument");
+  return DISPATCH_T(h, gz_condense_t<double>(h, (const double*)w, (double*)rhs),
+                    gz_condense_t<float>(h, (const float*)w, (float*)rhs));
+}
+
+int csipm_gondzio_unpack_solution(csipm_op_t h, void* w, const void* dx) {
+  if (!h || !w || !dx) return fail(CSIPM_ERR_ARG, "null argument");
+  return DISPATCH_T(h, gz_unpack_t<double>(h, (double*)w, (const double*)dx),
+                    gz_unpack_t<float>(h, (float*)w, (const float*)dx));
+}
+
+int csipm_gondzio_cg_solve(csipm_op_t h, const void* b, void* x, const void* Wp, const void* Wq,
+                           const void* P11, const void* P12, const void* P22, double atol,
+                           double rtol, int64_t itmax, int64_t* iters, double* resid,
+                           int* status) {
+  if (!h || !b || !x || !Wp || !Wq || !P11 || !P12 || !P22 || !iters || !resid || !status)
+    return fail(CSIPM_ERR_ARG, "null argument");
+  return DISPATCH_T(
+      h,
+      gz_cg_t<double>(h, (const double*)b, (double*)x, (const double*)Wp, (const double*)Wq,
+                      (const double*)P11, (const double*)P12, (const double*)P22, atol, rtol,
+                      itmax, iters, resid, status),
+      gz_cg_t<float>(h, (const float*)b, (float*)x, (const float*)Wp, (const float*)Wq,
+                     (const float*)P11, (const float*)P12, (const float*)P22, atol, rtol, itmax,
+                     iters, resid, status));
+}
+
+int csipm_gondzio_solve(csipm_op_t h, const double* z0, double lambda, double tol,
+                        int64_t max_iter, double cg_rtol, int64_t cg_itmax, double* x_out,
+                        double* y_out, double* info) {
+  if (!h || !z0 || !x_out || !y_out || !info) return fail(CSIPM_ERR_ARG, "null argument");
+  if (!(lambda >= 0) || !(tol > 0) || max_iter < 0)
+    return fail(CSIPM_ERR_ARG, "bad lambda / tol / max_iter");
+  if (h->prec != CSIPM_F64 && h->prec != CSIPM_MIXED)
+    return fail(CSIPM_ERR_UNSUPPORTED, "csipm_gondzio_solve needs a CSIPM_F64 or CSIPM_MIXED handle");
+  const int64_t n = h->N;
+  // the same central path as the FFT model under p = (c + beta)/2,
+  // q = (c - beta)/2 (slacks c + beta = 2p, c - beta = 2q)
+  double* beta = x_out + 2 * n;   // r's place holds beta until r is formed
+  double* c = y_out;
+  const int rc = h->prec == CSIPM_F64
+                     ? solve_solo<double>(h, z0, lambda, tol, max_iter, cg_rtol, cg_itmax, beta,
+                                          info, c)
+                     : solve_solo<float>(h, z0, lambda, tol, max_iter, cg_rtol, cg_itmax, beta,
+                                         info, c);
+  if (rc) return rc;
+  const double secs = info[6];
+  CHK(each(h, n, [=] __device__(int64_t i) {
+    x_out[i] = 0.5 * (c[i] + beta[i]);
+    x_out[n + i] = 0.5 * (c[i] - beta[i]);
+  }));
+  // r = z0 - M (p - q), y = r (stationarity in r of MadNLP's Lagrangian)
+  double *d, *md;
+  CHK(gz_scratch<double>(h, &d, &md));
+  CHK(each(h, n, [=] __device__(int64_t i) { d[i] = x_out[i] - x_out[n + i]; }));
+  CHK(op_m<double>(h, d, md));
+  CHK(each(h, n, [=] __device__(int64_t i) {
+    const double r = z0[i] - md[i];
+    x_out[2 * n + i] = r;
+    y_out[i] = r;
+  }));
+  double f = 0;
+  CHK(gz_obj_t<double>(h, x_out, lambda, &f));
+  info[0] = f;
+  info[6] = secs;
   return CSIPM_OK;
 }
 

@@ -31,15 +31,15 @@ int to_u(csipm_op* h, const T* beta, T* u, hipStream_t st, int sign = 1) {
 }
 
 template <typename T>
-int from_u(csipm_op* h, const T* u, T* beta, hipStream_t st) {
+int from_u(csipm_op* h, const T* u, T* beta, hipStream_t st, int sign = 1) {
   const int grid = (int)((h->Nc + kNT - 1) / kNT);
   const cpx<T>* uu = (const cpx<T>*)u;
   if (h->dim == 1)
-    hipLaunchKernelGGL((k_u_to_beta<1, T>), dim3(grid), dim3(kNT), 0, st, uu, beta, h->g, h->H1, h->Nc);
+    hipLaunchKernelGGL((k_u_to_beta<1, T>), dim3(grid), dim3(kNT), 0, st, uu, beta, h->g, h->H1, h->Nc, sign);
   else if (h->dim == 2)
-    hipLaunchKernelGGL((k_u_to_beta<2, T>), dim3(grid), dim3(kNT), 0, st, uu, beta, h->g, h->H1, h->Nc);
+    hipLaunchKernelGGL((k_u_to_beta<2, T>), dim3(grid), dim3(kNT), 0, st, uu, beta, h->g, h->H1, h->Nc, sign);
   else
-    hipLaunchKernelGGL((k_u_to_beta<3, T>), dim3(grid), dim3(kNT), 0, st, uu, beta, h->g, h->H1, h->Nc);
+    hipLaunchKernelGGL((k_u_to_beta<3, T>), dim3(grid), dim3(kNT), 0, st, uu, beta, h->g, h->H1, h->Nc, sign);
   HIPCHK(hipGetLastError());
   return CSIPM_OK;
 }
@@ -224,6 +224,7 @@ struct Engine {
     double lam = 0;
     const double* z0 = nullptr;
     double* beta_out = nullptr;
+    double* c_out = nullptr;  // optional: the final c iterate (reference order)
   };
   std::vector<Inst> inst;
   std::vector<std::pair<int64_t, int64_t>> freelist;   // (offset, length)
@@ -989,6 +990,7 @@ int engine_step(Engine* E, double tol, int max_iter, double cg_rtol, int* runnin
 int engine_harvest(Engine* E, int id, double* info) {
   Engine::Inst& in = E->inst[(size_t)id];
   CHK(from_u<double>(in.h, E->W(V_BETA) + in.off, in.beta_out, E->stream));
+  if (in.c_out) CHK(from_u<double>(in.h, E->W(V_C) + in.off, in.c_out, E->stream, 0));
   HIPCHK(hipMemcpyAsync(E->flags_h + id, E->flags + id, sizeof(BFlags), hipMemcpyDeviceToHost,
                         E->stream));
   HIPCHK(hipStreamSynchronize(E->stream));

@@ -1060,7 +1060,7 @@ __global__ __launch_bounds__(256) void k_beta_to_u(const T* __restrict__ beta, c
 
 template <int D, typename T>
 __global__ __launch_bounds__(256) void k_u_to_beta(const cpx<T>* __restrict__ u, T* beta,
-                                                   MapGeom g, int H1, int64_t nc) {
+                                                   MapGeom g, int H1, int64_t nc, int sign) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= nc) return;
   const int k1 = (int)(e % H1);
@@ -1072,7 +1072,7 @@ __global__ __launch_bounds__(256) void k_u_to_beta(const cpx<T>* __restrict__ u,
   if (scatter_targets<D>(k1, k2, k3, g, re, im, vertex, neg)) {
     const cpx<T> w = u[e];
     beta[re] = w.x;
-    if (!vertex) beta[im] = neg ? -w.y : w.y;
+    if (!vertex) beta[im] = (neg && sign) ? -w.y : w.y;
   }
 }
 

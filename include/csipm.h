@@ -202,6 +202,66 @@ int csipm_admm(csipm_op_t h, const double* z0, double lambda, double rho,
                int64_t maxt, double tol, double* beta_out, double* info);
 
 /* ------------------------------------------------------------------------
+ * Gondzio formulation (src/gondzio_model.jl, src/gondzio_kkt.jl): variables
+ * x = [p; q; r] (3N), beta = p - q, p, q >= 0, r free, one equality block
+ * c(x) = z0 - M (p - q) - r = 0, objective 1/2 r'r + lambda sum(p + q).
+ * M = csipm_m_perp_beta, M^T = csipm_m_perpt_z (no mask, as the reference).
+ * All vectors T* on the device (T = the handle's operator precision).
+ * ------------------------------------------------------------------------ */
+/* f = 1/2 r'r + lambda sum(p + q)  (gondzio_model.jl:67-79), synchronous.  */
+int csipm_gondzio_obj(csipm_op_t h, const void* x, double lambda, double* f);
+/* g = [lambda; lambda; r]  (gondzio_model.jl:81-94).                      */
+int csipm_gondzio_grad(csipm_op_t h, const void* x, double lambda, void* g);
+/* c = z0 - M (p - q) - r  (gondzio_model.jl:96-110).                      */
+int csipm_gondzio_cons(csipm_op_t h, const void* x, const void* z0, void* c);
+/* J^T v = [-M^T v; M^T v; -v]  (gondzio_kkt.jl:243-262).                  */
+int csipm_gondzio_jtprod(csipm_op_t h, const void* v, void* out);
+/* y = alpha K x + beta y on 2N vectors, K = [[M^T M + Wp, -M^T M],
+ * [-M^T M, M^T M + Wq]]: CondensedGondzioKKTSystem mul!
+ * (gondzio_kkt.jl:28-53).                                                  */
+int csipm_gondzio_kkt_apply(csipm_op_t h, const void* x, void* y, const void* Wp, const void* Wq,
+                            double alpha, double beta);
+/* The part of MadNLP.mul! that GondzioKKTSystem adds on 4N vectors
+ * [p; q; r; y] (gondzio_kkt.jl:203-241, without MadNLP._kktmul!'s
+ * diagonal terms).                                                        */
+int csipm_gondzio_kkt_mul_full(csipm_op_t h, const void* x, void* y, double alpha, double beta);
+/* build_kkt! (gondzio_kkt.jl:272-296): P = [[1 + Wp, -1], [-1, 1 + Wq]]^-1
+ * from the primal diagonals Wp, Wq (pr_diag), evaluated in the closed form
+ * P11 = (1 + Wq)/det, P12 = 1/det, P22 = (1 + Wp)/det, det = Wp + Wq + Wp Wq
+ * (the reference's inv/S sequence forms the same matrix).  The
+ * preconditioner product (GondzioPreconditioner mul!, :80-92) is
+ * csipm_precond_apply.                                                    */
+int csipm_gondzio_precond_build(csipm_op_t h, const void* Wp, const void* Wq, void* P11,
+                                void* P12, void* P22);
+/* solve_kkt! pieces (gondzio_kkt.jl:310-362): rhs (2N) = [w_p - M^T(w_r +
+ * w_y); w_q + M^T(w_r + w_y)] from w = [w_p; w_q; w_r; w_y] (4N); unpack:
+ * w_x = dx, dr = -w_y - M (dp - dq), dy = dr - w_r, in place.            */
+int csipm_gondzio_condense_rhs(csipm_op_t h, const void* w, void* rhs);
+int csipm_gondzio_unpack_solution(csipm_op_t h, void* w, const void* dx);
+/* PCG on the condensed Gondzio K (Krylov.krylov_solve! at
+ * gondzio_kkt.jl:339, the reference passes atol = 1e-12, rtol = 0), with
+ * csipm_cg_solve's conventions and outputs.  Runs as csipm_cg_solve on the
+ * congruent condensed FFT system (x = A y, A = 1/2 [[I, I], [-I, I]]):
+ * the same iterates and stopping point in exact arithmetic.  Synchronous. */
+int csipm_gondzio_cg_solve(csipm_op_t h, const void* b, void* x, const void* Wp, const void* Wq,
+                           const void* P11, const void* P12, const void* P22, double atol,
+                           double rtol, int64_t itmax, int64_t* iters, double* resid,
+                           int* status);
+/* Full solve of GondzioNLPModel (ipm_solve! with kkt_system =
+ * GondzioKKTSystem, src/CompressedSensingIPM.jl:31-36) by the library's
+ * interior-point driver: the Gondzio problem is the FFT model under
+ * p = (c + beta)/2, q = (c - beta)/2, with the same central path, so the
+ * driver solves that and returns
+ *   x_out  device, 3N doubles [p; q; r], r = z0 - M (p - q)
+ *   y_out  device, N doubles: the multiplier of c(x) = 0 (= r at a
+ *          solution, MadNLP's sign convention grad f + J^T y = z)
+ *   info   as csipm_solve, info[0] = 1/2 r'r + lambda sum(p + q).
+ * Handle precision CSIPM_F64 or CSIPM_MIXED.  Synchronous.               */
+int csipm_gondzio_solve(csipm_op_t h, const double* z0, double lambda, double tol,
+                        int64_t max_iter, double cg_rtol, int64_t cg_itmax, double* x_out,
+                        double* y_out, double* info);
+
+/* ------------------------------------------------------------------------
  * Batch engine: many solves of uneven sizes and dimensionalities in the same
  * kernel launches (one launch per pass / vector step for all instances),
  * with per-instance convergence masks, and continuous batching: instances
