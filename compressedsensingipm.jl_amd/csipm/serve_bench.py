@@ -114,6 +114,12 @@ def main():
                                 "makespan_s": round(span, 2), "p50_ms": pct(lat, 50),
                                 "p99_ms": pct(lat, 99),
                                 "all_solved": all(r.status == "solved" for r in res_b.values()),
+                                "not_solved": [[k, list(shapes[k]), r.status, r.iterations,
+                                                r.cg_iterations]
+                                               for k, r in sorted(res_b.items())
+                                               if r.status != "solved"][:10],
+                                "max_cg_iterations": max(r.cg_iterations
+                                                         for r in res_b.values()),
                                 "engine": {k: round(v, 3) for k, v in engine_stats.items()}}
         print(f"# batched: {span:.1f} s", file=sys.stderr, flush=True)
 

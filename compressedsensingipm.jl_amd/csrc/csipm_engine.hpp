@@ -207,7 +207,8 @@ struct Engine {
   int graph_nb = 0, graph_rel = -1;
   int64_t graph_version = -1;
   int64_t tables_version = 0;
-  std::vector<int64_t> table_sig;  // (id, handle, offset) of the set the tables were built for
+  std::vector<int64_t> table_sig;  // (id, admission, offset) of the set the tables were built for
+  int64_t admissions = 0;
   int table_nb = 0;
   bool table_ipm = false;
   // host-side seconds and counts, for csipm_batch_stats
@@ -225,6 +226,7 @@ struct Engine {
     const double* z0 = nullptr;
     double* beta_out = nullptr;
     double* c_out = nullptr;  // optional: the final c iterate (reference order)
+    int64_t serial = 0;       // admission number (unique over the engine's life)
   };
   std::vector<Inst> inst;
   std::vector<std::pair<int64_t, int64_t>> freelist;   // (offset, length)
@@ -610,8 +612,11 @@ template <typename TC>
 int engine_tables(Engine* E, int nb, bool ipm) {
   std::vector<int64_t> sig;
   for (int id : E->active) {
+    // the admission number, not the handle address: a handle created after
+    // another was destroyed can reuse its address while its device buffers
+    // (scratch, twiddles, mask) live elsewhere
     sig.push_back(id);
-    sig.push_back((int64_t)(intptr_t)E->inst[(size_t)id].h);
+    sig.push_back(E->inst[(size_t)id].serial);
     sig.push_back(E->inst[(size_t)id].off);
   }
   if (E->tables_dirty && sig == E->table_sig && nb == E->table_nb && ipm == E->table_ipm &&
@@ -655,6 +660,11 @@ int engine_admit(Engine* E, csipm_op* h, const double* z0, double lam, double* b
     }
   if (k < 0) return fail(CSIPM_ERR_ALLOC, "engine full (%d instances)", E->maxinst);
   const int64_t len = slots_of(h);
+  // partial-sum capacity: 8 quantities per vector block, the CG apply's final pass
+  const int64_t pcap = len / kSlotAlign * kPartPerAlign;
+  const int gcg = E->tc_size == 8 ? cg_final_grid<double>(h, 1) : cg_final_grid<float>(h, 1);
+  if (8 * (int64_t)vblocks(h->U) > pcap || gcg > pcap)
+    return fail(CSIPM_ERR_UNSUPPORTED, "partial sums of instance exceed its share");
   int64_t off = 0;
   if (!slots_alloc(E, len, &off))
     return fail(CSIPM_ERR_ALLOC, "no free slot range of %lld", (long long)len);
@@ -662,16 +672,12 @@ int engine_admit(Engine* E, csipm_op* h, const double* z0, double lam, double* b
   in.h = h;
   in.off = off;
   in.len = len;
+  in.serial = ++E->admissions;
   in.live = true;
   in.fresh = z0 != nullptr;   // no problem data: the instance only runs CG
   in.lam = lam;
   in.z0 = z0;
   in.beta_out = beta_out;
-  // partial-sum capacity: 8 quantities per vector block, the CG apply's final pass
-  const int64_t pcap = len / kSlotAlign * kPartPerAlign;
-  const int gcg = E->tc_size == 8 ? cg_final_grid<double>(h, 1) : cg_final_grid<float>(h, 1);
-  if (8 * (int64_t)vblocks(h->U) > pcap || gcg > pcap)
-    return fail(CSIPM_ERR_UNSUPPORTED, "partial sums of instance exceed its share");
   BInst bi{};
   bi.off = off;
   bi.n = h->U;

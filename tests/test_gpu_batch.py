@@ -132,3 +132,36 @@ def test_graphs_match_eager(cuda, monkeypatch, mixed):
         assert a.iterations == b.iterations and a.cg_iterations == b.cg_iterations, p.name
         assert a.objective == b.objective, p.name
         assert torch.equal(a.beta, b.beta), p.name
+
+
+def test_readmission_after_destroy(cuda):
+    """A handle destroyed after its instance retired, and a new one admitted
+    into the same id and slots (its address may be the old one's): the
+    launch tables are rebuilt for the new admission, so the result is the new
+    problem's own (an engine keyed its tables on the handle address once and
+    ran a re-admitted instance on the freed handle's buffers)."""
+    import gc
+    from csipm.batch import BatchSolver, slots_of
+    print("readmission", flush=True)
+    import csipm.workloads as wl
+    pa = wl.batch_instance((237, 237), seed=31)
+    pb = wl.batch_instance((237, 237), seed=32)
+    op_a = _op(pa, True)
+    bs = BatchSolver(slots_of(op_a), max_instances=1, mixed=True)
+    for p, make in ((pa, None), (pb, True)):
+        op = op_a if make is None else _op(p, True)
+        assert bs.admit(op, p.z0, p.lam, tag=0) is not None
+        steps = 0
+        while bs.live:
+            bs.step(tol=1e-7)
+            steps += 1
+            for i in bs.finished():
+                r = bs.harvest(i)
+            assert steps < 300
+        _same(r, _alone(p, True, 1e-7), "readmitted" if make else "first")
+        if make is None:
+            del op, op_a
+            gc.collect()
+            spacer = torch.empty(1 << 22, device=cuda)    # moves later allocations
+    bs.close()
+    del spacer
