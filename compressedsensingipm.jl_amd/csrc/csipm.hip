@@ -188,14 +188,46 @@ static bool plan_axis(int L, FftPlan& p, bool& bs) {
   return factorize(M, p);
 }
 
-// exp(-2 pi i k / M), k < M/2, in extended precision rounded once to FP64
-static std::vector<std::complex<double>> host_roots(size_t M) {
-  std::vector<std::complex<double>> r(M / 2);
+// exp(-2 pi t i / L) for t < L (L a multiple of 4 or not), in extended
+// precision rounded once: the quarter-wave symmetries fill three quarters of
+// the table from the first (exact identities, fewer trig evaluations)
+static void unit_roots(int64_t L, std::vector<std::complex<double>>& r, int64_t count) {
+  r.resize((size_t)count);
   const long double pi = acosl(-1.0L);
-  for (size_t k = 0; k < M / 2; ++k) {
-    const long double ang = -2.0L * pi * (long double)k / (long double)M;
-    r[k] = std::complex<double>((double)cosl(ang), (double)sinl(ang));
+  auto eval = [&](int64_t t) {
+    const long double ang = -2.0L * pi * (long double)t / (long double)L;
+    return std::complex<double>((double)cosl(ang), (double)sinl(ang));
+  };
+  if (L % 4 != 0) {
+    for (int64_t t = 0; t < count; ++t)
+      r[(size_t)t] = (2 * t > L && L - t < count) ? std::conj(r[(size_t)(L - t)]) : eval(t);
+    return;
   }
+  const int64_t q = L / 4;
+  for (int64_t t = 0; t < count; ++t) {
+    if (t <= q) {
+      r[(size_t)t] = eval(t);
+    } else {
+      // w^(t) = w^(t - q) * w^q = w^(t - q) * (-i)
+      const std::complex<double> b = r[(size_t)(t - q)];
+      r[(size_t)t] = std::complex<double>(b.imag(), -b.real());
+    }
+  }
+}
+
+// exp(-2 pi i k / M), k < M/2; kept per M (M a power of two)
+static std::shared_ptr<const std::vector<std::complex<double>>> host_roots(size_t M) {
+  static std::mutex mu;
+  static std::unordered_map<size_t, std::shared_ptr<const std::vector<std::complex<double>>>> cache;
+  {
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = cache.find(M);
+    if (it != cache.end()) return it->second;
+  }
+  auto r = std::make_shared<std::vector<std::complex<double>>>();
+  unit_roots((int64_t)M, *r, (int64_t)(M / 2));
+  std::lock_guard<std::mutex> lock(mu);
+  if (M <= (size_t(1) << 22)) cache[M] = r;
   return r;
 }
 
@@ -241,25 +273,28 @@ static std::shared_ptr<const BluesteinHost> bluestein_host(int L, int M) {
   BluesteinHost& t = *tp;
   t.L = L;
   t.M = M;
-  const long double pi = acosl(-1.0L);
+  // chirp: q = n^2 mod 2L exactly, angle -pi q / L reduced to (-pi, pi]
   t.w.resize((size_t)L);
+  const double pi = 3.141592653589793238462643383279502884;
   for (int n = 0; n < L; ++n) {
-    const long long q = ((long long)n * n) % (2LL * L);
-    const long double ang = -pi * (long double)q / (long double)L;
-    t.w[(size_t)n] = std::complex<double>((double)cosl(ang), (double)sinl(ang));
+    long long q = ((long long)n * n) % (2LL * L);
+    if (q > L) q -= 2LL * L;
+    const double ang = -pi * ((double)q / (double)L);
+    t.w[(size_t)n] = std::complex<double>(std::cos(ang), std::sin(ang));
   }
-  const std::vector<std::complex<double>> roots = host_roots((size_t)M);
-  for (int dir = 0; dir < 2; ++dir) {
-    std::vector<std::complex<double>> b((size_t)M, 0.0);
-    for (int m = 0; m < L; ++m) {
-      const std::complex<double> v = dir == 0 ? std::conj(t.w[(size_t)m]) : t.w[(size_t)m];
-      b[(size_t)m] = v;
-      if (m > 0) b[(size_t)(M - m)] = v;
-    }
-    host_fft_pow2(b, roots);
-    for (auto& x : b) x /= (double)M;
-    (dir == 0 ? t.bf : t.bi).swap(b);
+  // kernel spectra: b is symmetric (b_{M-m} = b_m), so FFT(conj b) =
+  // conj(FFT(b)) and one transform gives both
+  const auto roots = host_roots((size_t)M);
+  std::vector<std::complex<double>> b((size_t)M, 0.0);
+  for (int m = 0; m < L; ++m) {
+    b[(size_t)m] = t.w[(size_t)m];
+    if (m > 0) b[(size_t)(M - m)] = t.w[(size_t)m];
   }
+  host_fft_pow2(b, *roots);
+  for (auto& x : b) x /= (double)M;
+  t.bf.resize(b.size());
+  for (size_t i = 0; i < b.size(); ++i) t.bf[i] = std::conj(b[i]);
+  t.bi.swap(b);
   std::lock_guard<std::mutex> lock(mu);
   last = tp;
   return last;
@@ -283,33 +318,72 @@ static int upload_bluestein(int L, int M, void** chirp, void** bfw, void** binv)
   return CSIPM_OK;
 }
 
-// exp(-2 pi i t / L), t < L, in the op precision.  Tables are shared by all
-// handles on a device and kept for the life of the process (one per length
-// and precision; a serving process reuses them across requests).
+// exp(-2 pi i t / L), t < L, in the op precision.  Tables are shared by the
+// handles on a device and counted: a table no handle holds stays cached
+// (a serving process meets the same lengths again) until the unheld tables
+// exceed kTwiddleKeepBytes, when the least recently used are freed.
+constexpr size_t kTwiddleKeepBytes = size_t(256) << 20;
+
+struct TwiddleEntry {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  int refs = 0;
+  uint64_t last = 0;
+};
+
+static std::mutex g_tw_mu;
+static std::unordered_map<int64_t, TwiddleEntry> g_tw;
+static uint64_t g_tw_clock = 0;
+
 template <typename T>
 static int upload_twiddles(int L, void** out) {
-  static std::mutex mu;
-  static std::unordered_map<int64_t, void*> cache;
   int dev = 0;
   HIPCHK(hipGetDevice(&dev));
   const int64_t key = ((int64_t)dev << 40) | ((int64_t)sizeof(T) << 32) | (int64_t)L;
-  std::lock_guard<std::mutex> lock(mu);
-  auto it = cache.find(key);
-  if (it != cache.end()) {
-    *out = it->second;
+  std::lock_guard<std::mutex> lock(g_tw_mu);
+  auto it = g_tw.find(key);
+  if (it != g_tw.end()) {
+    it->second.refs += 1;
+    it->second.last = ++g_tw_clock;
+    *out = it->second.ptr;
     return CSIPM_OK;
   }
+  std::vector<std::complex<double>> r;
+  unit_roots(L, r, L);
   std::vector<cpx<T>> h((size_t)L);
-  const long double two_pi = 2.0L * acosl(-1.0L);
-  for (int t = 0; t < L; ++t) {
-    const long double a = two_pi * (long double)t / (long double)L;
-    h[(size_t)t].x = (T)cosl(a);
-    h[(size_t)t].y = (T)(-sinl(a));
-  }
+  for (int t = 0; t < L; ++t) h[(size_t)t] = {(T)r[(size_t)t].real(), (T)r[(size_t)t].imag()};
   HIPCHK(hipMalloc(out, sizeof(cpx<T>) * (size_t)L));
   HIPCHK(hipMemcpy(*out, h.data(), sizeof(cpx<T>) * (size_t)L, hipMemcpyHostToDevice));
-  cache[key] = *out;
+  TwiddleEntry& e = g_tw[key];
+  e.ptr = *out;
+  e.bytes = sizeof(cpx<T>) * (size_t)L;
+  e.refs = 1;
+  e.last = ++g_tw_clock;
   return CSIPM_OK;
+}
+
+// a handle lets go of a table it got from upload_twiddles
+static void release_twiddles(void* p) {
+  if (!p) return;
+  std::lock_guard<std::mutex> lock(g_tw_mu);
+  size_t unheld = 0;
+  for (auto& kv : g_tw) {
+    if (kv.second.ptr == p && kv.second.refs > 0) {
+      kv.second.refs -= 1;
+      p = nullptr;   // one reference per call
+    }
+    if (kv.second.refs == 0) unheld += kv.second.bytes;
+  }
+  while (unheld > kTwiddleKeepBytes) {
+    auto victim = g_tw.end();
+    for (auto it = g_tw.begin(); it != g_tw.end(); ++it)
+      if (it->second.refs == 0 && (victim == g_tw.end() || it->second.last < victim->second.last))
+        victim = it;
+    if (victim == g_tw.end()) break;
+    unheld -= victim->second.bytes;
+    (void)hipFree(victim->second.ptr);
+    g_tw.erase(victim);
+  }
 }
 
 // smallest tile (lines per workgroup) whose rows span 64 bytes: with the
@@ -1179,11 +1253,17 @@ const char* csipm_last_error(void) { return g_err.c_str(); }
 
 int csipm_op_destroy(csipm_op_t h) {
   if (!h) return CSIPM_OK;
-  // twiddle tables (tw, twA, twB) are shared across handles and stay
-  for (int i = 0; i < 3; ++i)
-    for (PrecState* ps : {&h->p64, &h->p32})
+  // twiddle tables (tw, twA, twB) are shared across handles: released to
+  // the cache; the Bluestein tables are the handle's own
+  for (PrecState* ps : {&h->p64, &h->p32}) {
+    for (int i = 0; i < 3; ++i) {
+      release_twiddles(ps->tw[i]);
       for (void* b : {ps->chirp[i], ps->bfw[i], ps->binv[i]})
         if (b) (void)hipFree(b);
+    }
+    release_twiddles(ps->twA);
+    release_twiddles(ps->twB);
+  }
   delete h->solo;
   void* bufs[] = {h->miss, h->valid, h->buf, h->part, h->scratch, h->gz, h->dplan};
   for (void* b : bufs)
