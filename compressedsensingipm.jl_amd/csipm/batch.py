@@ -174,16 +174,19 @@ class Request:
 
 
 def serve(requests, make_op, mixed=True, pool_budget_bytes=200 << 30, max_instances=512,
-          tol=1e-6, max_iter=200, cg_rtol=1e-10, clock=time.perf_counter, on_done=None):
+          tol=1e-6, max_iter=200, cg_rtol=1e-10, clock=time.perf_counter, on_done=None,
+          prep_workers=4):
     """Continuous batching over an arrival stream.
 
     requests: iterable of Request (arrival time in seconds from the start,
-    problem with .dims .z0 .lam .missing); make_op(problem) -> FFTOperator.
-    A request is admitted once it has arrived and its slots fit the pool
-    (sized from pool_budget_bytes); every loop turn runs one interior-point
-    iteration of all admitted problems and retires the finished ones.
-    Returns the list of Results (tag = the request) with admit / finish
-    times relative to the start."""
+    problem with .dims .z0 .lam .missing); make_op(problem) -> FFTOperator,
+    called by `prep_workers` helper threads as requests arrive (0: inline).
+    A request is admitted once it has arrived, its handle is ready and its
+    slots fit the pool (sized from pool_budget_bytes); every loop turn runs
+    one interior-point iteration of all admitted problems and retires the
+    finished ones.  Returns the list of Results (tag = the request) with
+    admit / finish times relative to the start."""
+    from concurrent.futures import ThreadPoolExecutor
     slots_budget = max(1, pool_budget_bytes // max(1, pool_bytes(1, mixed)))
     pending = list(requests)
     heapq.heapify(pending)
@@ -191,27 +194,42 @@ def serve(requests, make_op, mixed=True, pool_budget_bytes=200 << 30, max_instan
     waiting = []
     out = []
     t_start = clock()
-    need_max = 0
-    ops = {}
+    ops, futs = {}, {}
+    dev = torch.cuda.current_device()
+
+    def prepare(problem):
+        with torch.cuda.device(dev):
+            return make_op(problem)
+
+    prep = ThreadPoolExecutor(max_workers=prep_workers) if prep_workers > 0 else None
     while pending or waiting or (bs is not None and bs.live):
         now = clock() - t_start
         while pending and pending[0].arrival <= now:
-            waiting.append(heapq.heappop(pending))
+            req = heapq.heappop(pending)
+            waiting.append(req)
+            if prep is not None:
+                futs[req.seq] = prep.submit(prepare, req.problem)
         if bs is None:
             bs = BatchSolver(slots_budget, max_instances=max_instances, mixed=mixed)
         # admit in arrival order while the pool has room
         still = []
         for req in waiting:
-            op = ops.get(req.seq) or make_op(req.problem)
-            ops[req.seq] = op
-            need_max = max(need_max, slots_of(op))
+            if req.seq not in ops:
+                f = futs.get(req.seq)
+                if f is not None and not f.done():
+                    still.append(req)
+                    continue
+                ops[req.seq] = futs.pop(req.seq).result() if f is not None else \
+                    make_op(req.problem)
+            op = ops[req.seq]
             i = bs.admit(op, req.problem.z0, req.problem.lam, tag=req)
             if i is None:
                 still.append(req)
         waiting = still
         if bs.live == 0:
-            if pending:
-                time.sleep(max(0.0, min(0.001, pending[0].arrival - (clock() - t_start))))
+            if pending or waiting:
+                nxt = pending[0].arrival - (clock() - t_start) if pending else 0.0
+                time.sleep(max(0.0002, min(0.001, nxt)))
             continue
         bs.step(tol, max_iter, cg_rtol)
         for i in bs.finished():
@@ -224,4 +242,6 @@ def serve(requests, make_op, mixed=True, pool_budget_bytes=200 << 30, max_instan
                 on_done(r)
     if bs is not None:
         bs.close()
+    if prep is not None:
+        prep.shutdown()
     return out

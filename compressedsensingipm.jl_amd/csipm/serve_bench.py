@@ -8,10 +8,11 @@ Requests follow the serving workload: Poisson arrivals at `rate` per second;
 (odd side [63, 191]); each generated from its own seed by the generator of
 configs 0 / 1 / 2 at its shape (generation happens before the clock starts).
 
-* batched: csipm.batch.serve -- a request is admitted as soon as it has
-  arrived and its device footprint fits the budget, into a running batch
-  engine (FP32 operator and CG, FP64 control); one interior-point iteration
-  of every admitted request per engine step; finished ones retire.
+* batched: csipm.batch.serve's loop -- a request's operator handle is made
+  by a helper thread when it arrives; it is admitted once that is ready and
+  its device footprint fits the budget, into a running batch engine (FP32
+  operator and CG, FP64 control); one interior-point iteration of every
+  admitted request per engine step; finished ones retire.
 * plain: the same requests in arrival order, one csipm_solve at a time on
   FP64 handles, each started when it has arrived and the previous is done.
 
@@ -39,6 +40,8 @@ def main():
     ap.add_argument("--pool-gb", type=float, default=16.0)
     ap.add_argument("--arms", default="batched,plain")
     ap.add_argument("--check", type=int, default=12, help="requests re-solved for agreement")
+    ap.add_argument("--prep-workers", type=int, default=4,
+                    help="threads making operator handles for arriving requests")
     args = ap.parse_args()
 
     import numpy as np
@@ -71,22 +74,38 @@ def main():
 
     res_b = {}
     if "batched" in args.arms:
+        from concurrent.futures import ThreadPoolExecutor
         pool_slots = int(args.pool_gb * 2**30) // pool_bytes(1, True)
         bs = BatchSolver(pool_slots, max_instances=1024, mixed=True)
         budget = args.budget_gb * 2**30
         waiting, live_bytes, ops = [], {}, {}
+        # operator handles (tables, mask, plans) are made by helper threads as
+        # requests arrive, beside the engine steps of the main thread
+        prep = ThreadPoolExecutor(max_workers=args.prep_workers)
+
+        def make_op(k):
+            p = probs[k]
+            with torch.cuda.device(dev):
+                return FFTOperator(p.N, len(p.dims), p.dims, p.missing, mixed=True)
+
+        futs = {}
         k_next = 0
         t_start = time.perf_counter()
         while k_next < len(probs) or waiting or bs.live:
             now = time.perf_counter() - t_start
             while k_next < len(probs) and arrivals[k_next] <= now:
                 waiting.append(k_next)
+                futs[k_next] = prep.submit(make_op, k_next)
                 k_next += 1
             still = []
             for k in waiting:
                 p = probs[k]
-                op = ops.get(k) or FFTOperator(p.N, len(p.dims), p.dims, p.missing, mixed=True)
-                ops[k] = op
+                if k not in ops:
+                    if not futs[k].done():
+                        still.append(k)
+                        continue
+                    ops[k] = futs.pop(k).result()
+                op = ops[k]
                 need = pool_bytes(slots_of(op), True) + 48 * p.N
                 if sum(live_bytes.values()) + need > budget or \
                         bs.admit(op, z0s[k], p.lam, tag=k) is None:
@@ -95,8 +114,8 @@ def main():
                 live_bytes[k] = need
             waiting = still
             if bs.live == 0:
-                if k_next < len(probs):
-                    time.sleep(max(0.0, min(0.002, arrivals[k_next] - now)))
+                if k_next < len(probs) or waiting:
+                    time.sleep(max(0.0002, min(0.002, arrivals[min(k_next, len(probs) - 1)] - now)))
                 continue
             bs.step(args.tol)
             for i in bs.finished():
@@ -109,6 +128,7 @@ def main():
         span = time.perf_counter() - t_start
         engine_stats = bs.stats()
         bs.close()
+        prep.shutdown()
         lat = [res_b[k].t_done - arrivals[k] for k in res_b]
         out["batched_mixed"] = {"solves_per_s": round(len(res_b) / span, 2),
                                 "makespan_s": round(span, 2), "p50_ms": pct(lat, 50),

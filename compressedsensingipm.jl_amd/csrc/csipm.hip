@@ -892,6 +892,12 @@ int kkt_pass_info(csipm_op* h, int layout, const char** names, double* bytes) {
   return 5;
 }
 
+// Stream capture of the engine's CG graphs and the allocating / copying
+// calls of handle creation and destruction exclude each other: a handle made
+// on another thread (a serving front end preparing requests) while a capture
+// is open would invalidate it.
+static std::mutex g_capture_mu;
+
 #include "csipm_engine.hpp"
 
 // the handle's batch-of-one engine, created on first use, sized for it
@@ -1251,8 +1257,16 @@ const char* csipm_version(void) { return CSIPM_VERSION; }
 
 const char* csipm_last_error(void) { return g_err.c_str(); }
 
+static void destroy_impl(csipm_op_t h);
+
 int csipm_op_destroy(csipm_op_t h) {
   if (!h) return CSIPM_OK;
+  std::lock_guard<std::mutex> lock(g_capture_mu);
+  destroy_impl(h);
+  return CSIPM_OK;
+}
+
+static void destroy_impl(csipm_op_t h) {
   // twiddle tables (tw, twA, twB) are shared across handles: released to
   // the cache; the Bluestein tables are the handle's own
   for (PrecState* ps : {&h->p64, &h->p32}) {
@@ -1269,7 +1283,6 @@ int csipm_op_destroy(csipm_op_t h) {
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   delete h;
-  return CSIPM_OK;
 }
 
 // 1D lines above this length run as four-step transforms
@@ -1444,10 +1457,11 @@ int csipm_op_create(csipm_op_t* out, int dim, const int64_t sizes[3], int rdft, 
   *out = nullptr;
   csipm_op* h = new (std::nothrow) csipm_op();
   if (!h) return fail(CSIPM_ERR_ALLOC, "out of host memory");
+  std::lock_guard<std::mutex> lock(g_capture_mu);
   const int rc = create_impl(h, dim, sizes, rdft, precision, missing, n_missing, stream);
   if (rc != CSIPM_OK) {
     std::string keep = g_err;
-    csipm_op_destroy(h);
+    destroy_impl(h);
     g_err = keep;
     return rc;
   }

@@ -799,6 +799,7 @@ int capture_cg_graphs(Engine* E, int nb, int reliable) {
   StatTimer tm(&E->stats[Engine::ST_CAPTURE]);
   E->stats[Engine::ST_NCAPTURE] += 1;
   E->release_graphs();
+  std::lock_guard<std::mutex> lock(g_capture_mu);
   for (int g = 0; g < (reliable > 0 ? 2 : 1); ++g) {
     hipGraph_t graph = nullptr;
     HIPCHK(hipStreamBeginCapture(E->stream, hipStreamCaptureModeRelaxed));

@@ -165,3 +165,18 @@ def test_readmission_after_destroy(cuda):
             spacer = torch.empty(1 << 22, device=cuda)    # moves later allocations
     bs.close()
     del spacer
+
+
+def test_serve_stream(cuda):
+    """csipm.batch.serve over a short arrival stream (handles made by helper
+    threads): every request solved, each result the same as alone."""
+    from csipm.batch import Request, serve
+    print("serve stream", flush=True)
+    probs = _problems()[:6]
+    reqs = [Request(0.002 * k, k, p) for k, p in enumerate(probs)]
+    out = serve(reqs, lambda p: _op(p, True), mixed=True, pool_budget_bytes=1 << 30,
+                tol=1e-7, prep_workers=2)
+    assert sorted(r.tag.seq for r in out) == list(range(len(probs)))
+    for r in out:
+        p = probs[r.tag.seq]
+        _same(r, _alone(p, True, 1e-7), p.name)
