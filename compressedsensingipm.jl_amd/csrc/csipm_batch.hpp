@@ -150,28 +150,13 @@ __global__ void bk_cg_start(BSeg vseg, const int* ids, const BInst* __restrict__
 }
 
 // after q = K p (its last pass left gcg partials of p'q): alpha = gamma / p'q
-__global__ void bk_cg_alpha(const int* ids, const BInst* __restrict__ info, CgState* cg,
-                            const double* part) {
-  const int inst = ids[blockIdx.x];
-  CgState& st = cg[inst];
-  if (st.done) return;
-  const BInst& I = info[inst];
-  const double pq = sum_partials(part + I.pbase, I.gcg);
-  if (threadIdx.x == 0) {
-    st.pq = pq;
-    if (!(pq > 0.0)) {
-      st.done = 1;
-      st.status = 2;
-    } else {
-      st.alpha = st.gamma / pq;
-    }
-  }
-}
-
-// x += alpha p; r -= alpha q; z = P r; partial r'z
+// (every workgroup sums the apply's p'q partials itself,
+// in the same order, so all agree; the first one records it and a zero
+// curvature), then x += alpha p; r -= alpha q; z = P r; partial r'z at
+// pbase + gcg, beside the p'q partials other workgroups are still reading
 template <typename T>
 __global__ __launch_bounds__(256) void bk_cg_update(BSeg seg, const BInst* __restrict__ info,
-                                                    const CgState* cg, T* x, T* r, T* z,
+                                                    CgState* cg, T* x, T* r, T* z,
                                                     const T* __restrict__ p,
                                                     const T* __restrict__ q,
                                                     const T* __restrict__ P11,
@@ -180,8 +165,27 @@ __global__ __launch_bounds__(256) void bk_cg_update(BSeg seg, const BInst* __res
                                                     double* part) {
   const BFlags* flags = nullptr;
   BSEG_BEGIN
-  if (cg[inst].done) return;
-  const T alpha = (T)cg[inst].alpha;
+  CgState& st = cg[inst];
+  if (st.done) return;
+  __shared__ double s_pq;
+  const double pq0 = sum_partials(part + I.pbase, I.gcg);
+  if (threadIdx.x == 0) s_pq = pq0;
+  __syncthreads();
+  const double pq = s_pq;
+  if (!(pq > 0.0)) {
+    if (lb == 0 && threadIdx.x == 0) {
+      st.pq = pq;
+      st.done = 1;
+      st.status = 2;
+    }
+    return;
+  }
+  const double ad = st.gamma / pq;
+  if (lb == 0 && threadIdx.x == 0) {
+    st.pq = pq;
+    st.alpha = ad;
+  }
+  const T alpha = (T)ad;
   double acc = 0.0;
   BFOR(i) {
     const int64_t j = i + I.off;
@@ -207,7 +211,7 @@ __global__ __launch_bounds__(256) void bk_cg_update(BSeg seg, const BInst* __res
     z[k] = zz;
     acc += (double)rb * zb + (double)rz * zz;
   }
-  store_partial(part + I.pbase, acc, lb);
+  store_partial(part + I.pbase + I.gcg, acc, lb);
 }
 
 // gamma <- r'z, beta = gamma_new / gamma_old, stopping tests
@@ -217,7 +221,8 @@ __global__ void bk_cg_gamma(BSeg vseg, const int* ids, const BInst* __restrict__
   CgState& st = cg[inst];
   if (st.done) return;
   const BInst& I = info[inst];
-  const double g = sum_partials(part + I.pbase, vseg.first[inst + 1] - vseg.first[inst]);
+  const double g =
+      sum_partials(part + I.pbase + I.gcg, vseg.first[inst + 1] - vseg.first[inst]);
   if (threadIdx.x == 0) {
     st.iters += 1;
     st.rnorm = sqrt(g > 0 ? g : 0.0);

@@ -566,7 +566,7 @@ int update_gcg(Engine* E) {
   for (int id : E->active) {
     Engine::Inst& in = E->inst[(size_t)id];
     const int g = cg_final_grid<TC>(in.h, m);
-    if (g > in.len / kSlotAlign * kPartPerAlign)
+    if (g + (int64_t)vblocks(in.h->U) > in.len / kSlotAlign * kPartPerAlign)
       return fail(CSIPM_ERR_UNSUPPORTED, "partial sums of instance exceed its share");
     E->info_h[(size_t)id].gcg = g;
   }
@@ -663,7 +663,8 @@ int engine_admit(Engine* E, csipm_op* h, const double* z0, double lam, double* b
   // partial-sum capacity: 8 quantities per vector block, the CG apply's final pass
   const int64_t pcap = len / kSlotAlign * kPartPerAlign;
   const int gcg = E->tc_size == 8 ? cg_final_grid<double>(h, 1) : cg_final_grid<float>(h, 1);
-  if (8 * (int64_t)vblocks(h->U) > pcap || gcg > pcap)
+  // (the apply's p'q partials and the CG update's r'z partials side by side)
+  if (8 * (int64_t)vblocks(h->U) > pcap || gcg + (int64_t)vblocks(h->U) > pcap)
     return fail(CSIPM_ERR_UNSUPPORTED, "partial sums of instance exceed its share");
   int64_t off = 0;
   if (!slots_alloc(E, len, &off))
@@ -773,7 +774,6 @@ int cg_iteration(Engine* E, int nb, bool reliable) {
   TC *P11 = E->C<TC>(C_P11), *P12 = E->C<TC>(C_P12), *P22 = E->C<TC>(C_P22);
   hipStream_t st = E->stream;
   CHK(run_chain<TC>(E->chains[CALL_CGA], st));
-  hipLaunchKernelGGL(bk_cg_alpha, dim3(na), dim3(kNT), 0, st, E->ids, E->info, E->cg, E->part);
   hipLaunchKernelGGL((bk_cg_update<TC>), dim3(E->vgrid), dim3(kNT), 0, st, E->vseg, E->info,
                      E->cg, x, r, z, p, q, P11, P12, P22, nb, E->part);
   hipLaunchKernelGGL(bk_cg_gamma, dim3(na), dim3(kNT), 0, st, E->vseg, E->ids, E->info, E->cg,
