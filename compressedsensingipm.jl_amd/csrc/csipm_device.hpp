@@ -422,6 +422,29 @@ __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirs
 // exact for a, d <= 2^16 (host checks the plan limits)
 __device__ __forceinline__ int fdiv(int a, uint32_t m) { return (int)__umulhi((uint32_t)a, m); }
 
+// 15-point DFT by the prime-factor (Good-Thomas) split 15 = 3 x 5, which
+// needs no twiddles: with x read at n = (5a + 3b) mod 15 and X written at
+// k = (10 k1 + 6 k2) mod 15, X = DFT3 over a of DFT5 over b (W15^(nk) =
+// W3^(a k1) W5^(b k2) for these maps).  3 five-point and 5 three-point
+// transforms instead of the 15-point pair form's 7 x 7 products.
+template <typename T, bool FWD>
+__device__ __forceinline__ void dft15_store(cpx<T>* v, cpx<T>* __restrict__ dst, int stride) {
+  cpx<T> y[3][5];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+#pragma unroll
+    for (int b = 0; b < 5; ++b) y[a][b] = v[(5 * a + 3 * b) % 15];
+    dft_odd<T, 5, FWD>(y[a]);
+  }
+#pragma unroll
+  for (int k2 = 0; k2 < 5; ++k2) {
+    cpx<T> z[3] = {y[0][k2], y[1][k2], y[2][k2]};
+    dft_odd<T, 3, FWD>(z);
+#pragma unroll
+    for (int k1 = 0; k1 < 3; ++k1) dst[((10 * k1 + 6 * k2) % 15) * stride] = z[k1];
+  }
+}
+
 template <typename T, int R, int E, int NT, bool FWD>
 __device__ __forceinline__ void stockham_stage(cpx<T>* __restrict__ lds, const FftPlan& p, int s,
                                                int lg, int ps, const cpx<T>* __restrict__ tw) {
@@ -486,7 +509,10 @@ __device__ __forceinline__ void stockham_stage(cpx<T>* __restrict__ lds, const F
 #pragma unroll
         for (int r = 0; r < R; ++r) dst[r * Ns * ps] = v[q][r];
       } else {
-        dft_odd_store<T, R, FWD>(v[q], dst, Ns * ps);
+        if constexpr (R == 15)
+          dft15_store<T, FWD>(v[q], dst, Ns * ps);
+        else
+          dft_odd_store<T, R, FWD>(v[q], dst, Ns * ps);
       }
     }
   }
