@@ -64,6 +64,7 @@ namespace {
 
 constexpr int kNT = 256;      // threads per workgroup of the FFT passes
 constexpr int kNTBig = 1024;  // dimension-1 passes with long lines (1D)
+constexpr int kNTSmall = 128;  // batched passes whose tile holds <= 128*kE elements
 constexpr int kE = 16;        // complex elements staged per thread and stage
 constexpr size_t kLdsSmall = 64 * 1024;
 constexpr size_t kLdsMax = 160 * 1024;
@@ -71,6 +72,7 @@ constexpr size_t kLdsMax = 160 * 1024;
 struct LineCfg {
   int lg = 0, ntile1 = 0, grid = 0;
   size_t lds = 0;
+  int nt = 256;   // workgroup size of the batched pass (kNT or kNTSmall)
 };
 struct Dim1Cfg {
   int nt = 0, lg = 0, ps = 0, npairs = 0, grid = 0;

@@ -97,27 +97,29 @@ int launch_spec(const PassSpec& s, int grid, size_t lds, hipStream_t st, const v
     return CSIPM_OK;                                                            \
   } while (0)
   if (s.kind == PK_LINES) {
-    if (NT != kNT) return fail(CSIPM_ERR_UNSUPPORTED, "line pass with %d threads", NT);
-    if (s.D == 2 && s.src == SRC_U && s.dst == DST_BUF && !s.fm)
-      CSIPM_LAUNCH((kb_lines<T, kNT, kE, 2, SRC_U, DST_BUF, false>));
-    if (s.D == 2 && s.src == SRC_BUF && s.dst == DST_U && s.fm)
-      CSIPM_LAUNCH((kb_lines<T, kNT, kE, 2, SRC_BUF, DST_U, true>));
-    if (s.D == 3 && s.src == SRC_U && s.dst == DST_BUF && !s.fm)
-      CSIPM_LAUNCH((kb_lines<T, kNT, kE, 3, SRC_U, DST_BUF, false>));
-    if (s.D == 3 && s.src == SRC_BUF && s.dst == DST_BUF && !s.fm)
-      CSIPM_LAUNCH((kb_lines<T, kNT, kE, 3, SRC_BUF, DST_BUF, false>));
-    if (s.D == 3 && s.src == SRC_BUF && s.dst == DST_BUF && s.fm)
-      CSIPM_LAUNCH((kb_lines<T, kNT, kE, 3, SRC_BUF, DST_BUF, true>));
-    if (s.D == 3 && s.src == SRC_BUF && s.dst == DST_U && s.fm)
-      CSIPM_LAUNCH((kb_lines<T, kNT, kE, 3, SRC_BUF, DST_U, true>));
+    if constexpr (NT == kNT || NT == kNTSmall) {
+      if (s.D == 2 && s.src == SRC_U && s.dst == DST_BUF && !s.fm)
+        CSIPM_LAUNCH((kb_lines<T, NT, kE, 2, SRC_U, DST_BUF, false>));
+      if (s.D == 2 && s.src == SRC_BUF && s.dst == DST_U && s.fm)
+        CSIPM_LAUNCH((kb_lines<T, NT, kE, 2, SRC_BUF, DST_U, true>));
+      if (s.D == 3 && s.src == SRC_U && s.dst == DST_BUF && !s.fm)
+        CSIPM_LAUNCH((kb_lines<T, NT, kE, 3, SRC_U, DST_BUF, false>));
+      if (s.D == 3 && s.src == SRC_BUF && s.dst == DST_BUF && !s.fm)
+        CSIPM_LAUNCH((kb_lines<T, NT, kE, 3, SRC_BUF, DST_BUF, false>));
+      if (s.D == 3 && s.src == SRC_BUF && s.dst == DST_BUF && s.fm)
+        CSIPM_LAUNCH((kb_lines<T, NT, kE, 3, SRC_BUF, DST_BUF, true>));
+      if (s.D == 3 && s.src == SRC_BUF && s.dst == DST_U && s.fm)
+        CSIPM_LAUNCH((kb_lines<T, NT, kE, 3, SRC_BUF, DST_U, true>));
+    }
+    return fail(CSIPM_ERR_UNSUPPORTED, "line pass with %d threads", NT);
   } else if (s.kind == PK_COL) {
-    if (NT != kNT) return fail(CSIPM_ERR_UNSUPPORTED, "column pass with %d threads", NT);
+    if constexpr (NT != kNT) return fail(CSIPM_ERR_UNSUPPORTED, "column pass with %d threads", NT);
     if (s.fm == C_FIRST) CSIPM_LAUNCH((kb_col<T, kNT, kE, SRC_U, DST_BUF, C_FIRST>));
     if (s.fm == C_REAL_IN) CSIPM_LAUNCH((kb_col<T, kNT, kE, SRC_REAL, DST_BUF, C_REAL_IN>));
     if (s.fm == C_MID) CSIPM_LAUNCH((kb_col<T, kNT, kE, SRC_BUF, DST_BUF, C_MID>));
     if (s.fm == C_LAST) CSIPM_LAUNCH((kb_col<T, kNT, kE, SRC_BUF, DST_U, C_LAST>));
   } else if (s.kind == PK_ROW) {
-    if (NT != kNT) return fail(CSIPM_ERR_UNSUPPORTED, "row pass with %d threads", NT);
+    if constexpr (NT != kNT) return fail(CSIPM_ERR_UNSUPPORTED, "row pass with %d threads", NT);
     if (s.fm == R_MTM) CSIPM_LAUNCH((kb_row<T, kNT, kE, R_MTM>));
     if (s.fm == R_R2C) CSIPM_LAUNCH((kb_row<T, kNT, kE, R_R2C>));
     if (s.fm == R_CONV_INV) CSIPM_LAUNCH((kb_row<T, kNT, kE, R_CONV_INV>));
@@ -170,6 +172,7 @@ int run_chain(const Chain& c, hipStream_t st) {
   for (const ChainSub& s : c.subs) {
     BSeg seg{s.seg, s.seg + s.grid};
     if (s.nt == kNT) CHK((launch_spec<T, kNT>(s.spec, s.grid, s.lds, st, s.tab, seg)));
+    else if (s.nt == kNTSmall) CHK((launch_spec<T, kNTSmall>(s.spec, s.grid, s.lds, st, s.tab, seg)));
     else CHK((launch_spec<T, kNTBig>(s.spec, s.grid, s.lds, st, s.tab, seg)));
   }
   return CSIPM_OK;
@@ -407,6 +410,13 @@ BatchCfg batch_cfg(csipm_op* h, int members) {
   b.d1 = dim1_cfg(h->plan[0].L, (int)(h->N / h->n[0]), cs, target);
   if (h->dim >= 2) b.lines[1] = line_cfg(h->H1, h->plan[1].L, h->n[2], cs, target);
   if (h->dim == 3) b.lines[2] = line_cfg(h->H1, h->plan[2].L, h->n[1], cs, target);
+  // a tile of at most kNTSmall*kE elements runs on kNTSmall threads: a stage
+  // of radix R has L*lines/R butterflies, and on 256 threads a 255-point
+  // tile of 8 lines (120 radix-17 butterflies) would leave half the lanes idle
+  auto small = [](int64_t elems) { return elems <= (int64_t)kNTSmall * kE; };
+  if (b.d1.nt == kNT && small((int64_t)h->plan[0].L << b.d1.lg)) b.d1.nt = kNTSmall;
+  for (int a = 1; a < h->dim; ++a)
+    b.lines[a].nt = small((int64_t)h->plan[a].L << b.lines[a].lg) ? kNTSmall : kNT;
   return b;
 }
 
@@ -471,6 +481,13 @@ int chain_call_args(Engine* E, int id, int call, int nb, const PassSpec& s,
   return CSIPM_OK;
 }
 
+// workgroup size of a pass of an instance with launch shapes bc
+inline int pass_nt(const PassSpec& s, const BatchCfg& bc) {
+  if (s.kind == PK_DIM1) return bc.d1.nt;
+  if (s.kind == PK_LINES) return bc.lines[s.axis].nt;
+  return kNT;
+}
+
 template <typename T>
 int build_chain(Engine* E, const std::vector<int>& ids, int call, int nb, Chain& out) {
   out.release();
@@ -485,14 +502,14 @@ int build_chain(Engine* E, const std::vector<int>& ids, int call, int nb, Chain&
                                specs);
     for (int p = 0; p < np; ++p) {
       const PassSpec& s = specs[p];
-      for (int nt : {kNT, kNTBig}) {
+      for (int nt : {kNTSmall, kNT, kNTBig}) {
         std::vector<PassArgs<T>> tab;
         std::vector<int> grids;
         size_t lds = 0;
         for (int id : mem) {
           csipm_op* h = E->inst[(size_t)id].h;
           const BatchCfg bc = batch_cfg<T>(h, (int)ids.size());
-          const int want = s.kind == PK_DIM1 ? bc.d1.nt : kNT;
+          const int want = pass_nt(s, bc);
           if (want != nt) continue;
           PassArgs<T> a{};
           CHK(chain_call_args<T>(E, id, call, nb, s, bc, a));
@@ -517,7 +534,7 @@ int build_chain(Engine* E, const std::vector<int>& ids, int call, int nb, Chain&
         sub.pass = p;
         for (int id : mem) {
           csipm_op* hm = E->inst[(size_t)id].h;
-          const int want = s.kind == PK_DIM1 ? batch_cfg<T>(hm, (int)ids.size()).d1.nt : kNT;
+          const int want = pass_nt(s, batch_cfg<T>(hm, (int)ids.size()));
           if (want == nt) sub.ids.push_back(id);
         }
         sub.nt = nt;
@@ -1052,6 +1069,8 @@ int engine_apply_timed(Engine* E, int64_t nrep, double* ms, double* bytes, std::
       const ChainSub& sb = c.subs[(size_t)k];
       BSeg seg{sb.seg, sb.seg + sb.grid};
       if (sb.nt == kNT) CHK((launch_spec<TC, kNT>(sb.spec, sb.grid, sb.lds, E->stream, sb.tab, seg)));
+      else if (sb.nt == kNTSmall)
+        CHK((launch_spec<TC, kNTSmall>(sb.spec, sb.grid, sb.lds, E->stream, sb.tab, seg)));
       else CHK((launch_spec<TC, kNTBig>(sb.spec, sb.grid, sb.lds, E->stream, sb.tab, seg)));
       HIPCHK(hipEventRecord(ev[(size_t)k + 1], E->stream));
     }
