@@ -51,6 +51,19 @@ static int fail(int code, const char* fmt, ...) {
                   __FILE__, __LINE__);                                                   \
   } while (0)
 
+// Stream capture of the engine's CG graphs and the device calls of handle
+// creation and destruction (allocations, copies, frees, small kernels)
+// exclude each other: such a call made on another thread (a serving front end
+// preparing requests) while a capture is open would invalidate the capture.
+// Host-side table work runs outside the lock.
+static std::mutex g_capture_mu;
+
+#define HIPL(expr)                                          \
+  do {                                                      \
+    std::lock_guard<std::mutex> lk_(g_capture_mu);          \
+    HIPCHK(expr);                                           \
+  } while (0)
+
 #define CHK(expr)                    \
   do {                               \
     int rc_ = (expr);                \
@@ -306,6 +319,7 @@ template <typename T>
 static int upload_cpx(const std::vector<std::complex<double>>& v, void** out) {
   std::vector<cpx<T>> h(v.size());
   for (size_t i = 0; i < v.size(); ++i) h[i] = {(T)v[i].real(), (T)v[i].imag()};
+  std::lock_guard<std::mutex> lk(g_capture_mu);
   HIPCHK(hipMalloc(out, sizeof(cpx<T>) * h.size()));
   HIPCHK(hipMemcpy(*out, h.data(), sizeof(cpx<T>) * h.size(), hipMemcpyHostToDevice));
   return CSIPM_OK;
@@ -342,49 +356,81 @@ static int upload_twiddles(int L, void** out) {
   int dev = 0;
   HIPCHK(hipGetDevice(&dev));
   const int64_t key = ((int64_t)dev << 40) | ((int64_t)sizeof(T) << 32) | (int64_t)L;
-  std::lock_guard<std::mutex> lock(g_tw_mu);
-  auto it = g_tw.find(key);
-  if (it != g_tw.end()) {
-    it->second.refs += 1;
-    it->second.last = ++g_tw_clock;
-    *out = it->second.ptr;
-    return CSIPM_OK;
+  {
+    std::lock_guard<std::mutex> lock(g_tw_mu);
+    auto it = g_tw.find(key);
+    if (it != g_tw.end()) {
+      it->second.refs += 1;
+      it->second.last = ++g_tw_clock;
+      *out = it->second.ptr;
+      return CSIPM_OK;
+    }
   }
+  // a new table: computed and uploaded without holding the cache lock
   std::vector<std::complex<double>> r;
   unit_roots(L, r, L);
   std::vector<cpx<T>> h((size_t)L);
   for (int t = 0; t < L; ++t) h[(size_t)t] = {(T)r[(size_t)t].real(), (T)r[(size_t)t].imag()};
-  HIPCHK(hipMalloc(out, sizeof(cpx<T>) * (size_t)L));
-  HIPCHK(hipMemcpy(*out, h.data(), sizeof(cpx<T>) * (size_t)L, hipMemcpyHostToDevice));
-  TwiddleEntry& e = g_tw[key];
-  e.ptr = *out;
-  e.bytes = sizeof(cpx<T>) * (size_t)L;
-  e.refs = 1;
-  e.last = ++g_tw_clock;
+  void* p = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_capture_mu);
+    HIPCHK(hipMalloc(&p, sizeof(cpx<T>) * (size_t)L));
+    const hipError_t e = hipMemcpy(p, h.data(), sizeof(cpx<T>) * (size_t)L, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      (void)hipFree(p);
+      return fail(CSIPM_ERR_HIP, "twiddle upload: %s", hipGetErrorString(e));
+    }
+  }
+  void* dup = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_tw_mu);
+    TwiddleEntry& e = g_tw[key];
+    if (e.ptr) {
+      dup = p;            // another thread made the same table meanwhile
+    } else {
+      e.ptr = p;
+      e.bytes = sizeof(cpx<T>) * (size_t)L;
+    }
+    e.refs += 1;
+    e.last = ++g_tw_clock;
+    *out = e.ptr;
+  }
+  if (dup) {
+    std::lock_guard<std::mutex> lk(g_capture_mu);
+    (void)hipFree(dup);
+  }
   return CSIPM_OK;
 }
 
 // a handle lets go of a table it got from upload_twiddles
 static void release_twiddles(void* p) {
   if (!p) return;
-  std::lock_guard<std::mutex> lock(g_tw_mu);
-  size_t unheld = 0;
-  for (auto& kv : g_tw) {
-    if (kv.second.ptr == p && kv.second.refs > 0) {
-      kv.second.refs -= 1;
-      p = nullptr;   // one reference per call
+  std::vector<void*> victims;
+  {
+    std::lock_guard<std::mutex> lock(g_tw_mu);
+    size_t unheld = 0;
+    for (auto& kv : g_tw) {
+      if (kv.second.ptr == p && kv.second.refs > 0) {
+        kv.second.refs -= 1;
+        p = nullptr;   // one reference per call
+      }
+      if (kv.second.refs == 0) unheld += kv.second.bytes;
     }
-    if (kv.second.refs == 0) unheld += kv.second.bytes;
+    while (unheld > kTwiddleKeepBytes) {
+      auto victim = g_tw.end();
+      for (auto it = g_tw.begin(); it != g_tw.end(); ++it)
+        if (it->second.refs == 0 &&
+            (victim == g_tw.end() || it->second.last < victim->second.last))
+          victim = it;
+      if (victim == g_tw.end()) break;
+      unheld -= victim->second.bytes;
+      victims.push_back(victim->second.ptr);
+      g_tw.erase(victim);
+    }
   }
-  while (unheld > kTwiddleKeepBytes) {
-    auto victim = g_tw.end();
-    for (auto it = g_tw.begin(); it != g_tw.end(); ++it)
-      if (it->second.refs == 0 && (victim == g_tw.end() || it->second.last < victim->second.last))
-        victim = it;
-    if (victim == g_tw.end()) break;
-    unheld -= victim->second.bytes;
-    (void)hipFree(victim->second.ptr);
-    g_tw.erase(victim);
+  if (!victims.empty()) {
+    std::lock_guard<std::mutex> lk(g_capture_mu);
+    for (void* v : victims) (void)hipFree(v);
   }
 }
 
@@ -894,12 +940,6 @@ int kkt_pass_info(csipm_op* h, int layout, const char** names, double* bytes) {
   return 5;
 }
 
-// Stream capture of the engine's CG graphs and the allocating / copying
-// calls of handle creation and destruction exclude each other: a handle made
-// on another thread (a serving front end preparing requests) while a capture
-// is open would invalidate it.
-static std::mutex g_capture_mu;
-
 #include "csipm_engine.hpp"
 
 // the handle's batch-of-one engine, created on first use, sized for it
@@ -1263,23 +1303,23 @@ static void destroy_impl(csipm_op_t h);
 
 int csipm_op_destroy(csipm_op_t h) {
   if (!h) return CSIPM_OK;
-  std::lock_guard<std::mutex> lock(g_capture_mu);
   destroy_impl(h);
   return CSIPM_OK;
 }
 
 static void destroy_impl(csipm_op_t h) {
   // twiddle tables (tw, twA, twB) are shared across handles: released to
-  // the cache; the Bluestein tables are the handle's own
+  // the cache; the Bluestein tables and the rest are the handle's own
   for (PrecState* ps : {&h->p64, &h->p32}) {
-    for (int i = 0; i < 3; ++i) {
-      release_twiddles(ps->tw[i]);
-      for (void* b : {ps->chirp[i], ps->bfw[i], ps->binv[i]})
-        if (b) (void)hipFree(b);
-    }
+    for (int i = 0; i < 3; ++i) release_twiddles(ps->tw[i]);
     release_twiddles(ps->twA);
     release_twiddles(ps->twB);
   }
+  std::lock_guard<std::mutex> lk(g_capture_mu);
+  for (PrecState* ps : {&h->p64, &h->p32})
+    for (int i = 0; i < 3; ++i)
+      for (void* b : {ps->chirp[i], ps->bfw[i], ps->binv[i]})
+        if (b) (void)hipFree(b);
   delete h->solo;
   void* bufs[] = {h->miss, h->valid, h->buf, h->part, h->scratch, h->gz, h->dplan};
   for (void* b : bufs)
@@ -1405,8 +1445,8 @@ static int create_impl(csipm_op* h, int dim, const int64_t sizes[3], int rdft, i
   }
   {
     const FftPlan plans[5] = {h->plan[0], h->plan[1], h->plan[2], h->planA, h->planB};
-    HIPCHK(hipMalloc(&h->dplan, sizeof(plans)));
-    HIPCHK(hipMemcpy(h->dplan, plans, sizeof(plans), hipMemcpyHostToDevice));
+    HIPL(hipMalloc(&h->dplan, sizeof(plans)));
+    HIPL(hipMemcpy(h->dplan, plans, sizeof(plans), hipMemcpyHostToDevice));
   }
   if (h->has64()) CHK(set_shapes(h, h->p64, sizeof(cpx<double>)));
   if (h->has32()) CHK(set_shapes(h, h->p32, sizeof(cpx<float>)));
@@ -1424,14 +1464,15 @@ static int create_impl(csipm_op* h, int dim, const int64_t sizes[3], int rdft, i
                     (long long)h->N);
       bits[(size_t)(s >> 5)] |= 1u << (s & 31);
     }
-    HIPCHK(hipMalloc(&h->miss, sizeof(uint32_t) * (size_t)words));
-    HIPCHK(hipMemcpy(h->miss, bits.data(), sizeof(uint32_t) * (size_t)words,
+    HIPL(hipMalloc(&h->miss, sizeof(uint32_t) * (size_t)words));
+    HIPL(hipMemcpy(h->miss, bits.data(), sizeof(uint32_t) * (size_t)words,
                      hipMemcpyHostToDevice));
   }
   // u-slot validity
   const int64_t vwords = (h->U + 31) / 32;
-  HIPCHK(hipMalloc(&h->valid, sizeof(uint32_t) * (size_t)vwords));
+  HIPL(hipMalloc(&h->valid, sizeof(uint32_t) * (size_t)vwords));
   {
+    std::lock_guard<std::mutex> lk(g_capture_mu);
     const int grid = (int)((vwords + kNT - 1) / kNT);
     if (dim == 1)
       hipLaunchKernelGGL((k_u_valid<1>), dim3(grid), dim3(kNT), 0, h->stream, h->valid, h->g, h->H1, h->Nc);
@@ -1441,15 +1482,15 @@ static int create_impl(csipm_op* h, int dim, const int64_t sizes[3], int rdft, i
       hipLaunchKernelGGL((k_u_valid<3>), dim3(grid), dim3(kNT), 0, h->stream, h->valid, h->g, h->H1, h->Nc);
     HIPCHK(hipGetLastError());
   }
-  HIPCHK(hipMalloc(&h->buf, (size_t)std::max<int64_t>(h->Nc, h->bF) * sizeof(cpx<double>)));
+  HIPL(hipMalloc(&h->buf, (size_t)std::max<int64_t>(h->Nc, h->bF) * sizeof(cpx<double>)));
   int np = 4096;
   for (PrecState* ps : {&h->p64, &h->p32}) {
     np = std::max(np, ps->d1.grid);
     for (int ax = 1; ax < dim; ++ax) np = std::max(np, ps->lines[ax].grid);
   }
   h->npart = 8 * np;   // up to 8 reduced quantities per pass
-  HIPCHK(hipMalloc(&h->part, sizeof(double) * (size_t)h->npart));
-  HIPCHK(hipStreamSynchronize(h->stream));
+  HIPL(hipMalloc(&h->part, sizeof(double) * (size_t)h->npart));
+  HIPL(hipStreamSynchronize(h->stream));
   return CSIPM_OK;
 }
 
@@ -1459,7 +1500,6 @@ int csipm_op_create(csipm_op_t* out, int dim, const int64_t sizes[3], int rdft, 
   *out = nullptr;
   csipm_op* h = new (std::nothrow) csipm_op();
   if (!h) return fail(CSIPM_ERR_ALLOC, "out of host memory");
-  std::lock_guard<std::mutex> lock(g_capture_mu);
   const int rc = create_impl(h, dim, sizes, rdft, precision, missing, n_missing, stream);
   if (rc != CSIPM_OK) {
     std::string keep = g_err;
