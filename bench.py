@@ -1,11 +1,11 @@
 """Benchmark: compressed-sensing IPM solves on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload config1] [--batch B]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload 2d255] [--batch B]
 
-Workload config1 (default): B instances of the 2D 255x255 generator (40
-modes, r=3 discs on a 16-lattice, lambda 0.05, tol 1e-8, instance seeds
-1..B), already resident in HBM.  (config0: B 1D N=1024 instances; config2:
-one 3D 255^3 instance; config3: the 384 uneven 1D/2D instances at tol 1e-6.)
+Workload 2d255 (default): B instances of the 2D 255x255 generator (40 modes,
+r=3 discs on a 16-lattice, lambda 0.05, tol 1e-8, instance seeds 1..B),
+already resident in HBM.  (1d1024: B 1D N=1024 instances; 3d255: one 3D
+255^3 instance; uneven: the 384 uneven 1D/2D instances at tol 1e-6.)
 A step solves all B:
 
 * batched mixed (the measured form, `value`): one batch engine, all B
@@ -43,23 +43,24 @@ for _p in (ROOT, PKG):
 HBM_PEAK_GBS = 8000.0
 
 WORKLOADS = {
-    "config0": ("config0", 1e-8, "1D N=1024, 12 modes, gaps+5% dropout, lambda 0.1, tol 1e-8"),
-    "config1": ("config1", 1e-8,
+    "1d1024": ("signal_1d_1024", 1e-8,
+               "1D N=1024, 12 modes, gaps+5% dropout, lambda 0.1, tol 1e-8"),
+    "2d255": ("punched_2d_255", 1e-8,
                 "2D 255x255, 40 modes, r=3 discs on a 16-lattice, lambda 0.05, tol 1e-8"),
-    "config2": ("config2", 1e-8,
+    "3d255": ("punched_3d", 1e-8,
                 "3D 255^3, 200 modes, r=2 balls on an 8-lattice + 1% dropout, lambda 0.05, "
                 "tol 1e-8"),
-    "config3": ("config3", 1e-6,
+    "uneven": ("uneven_batch", 1e-6,
                 "256 1D (N log-uniform [512, 65536]) + 128 2D (odd side [63, 511]) instances, "
                 "10% dropout + gaps/discs, lambda 0.1, tol 1e-6"),
 }
 
 
 def make_problems(wl, workload, batch):
-    if workload == "config3":
-        return wl.config3()
-    if workload == "config2":
-        return [wl.config2(instance=1)]
+    if workload == "uneven":
+        return wl.uneven_batch()
+    if workload == "3d255":
+        return [wl.punched_3d(instance=1)]
     gen = getattr(wl, WORKLOADS[workload][0])
     return [gen(instance=i + 1) for i in range(batch)]
 
@@ -120,13 +121,13 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="config1", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="2d255", choices=sorted(WORKLOADS))
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every "
-              "instance fits one MI355X; sharding is described in DESIGN.md)", file=sys.stderr)
+              "instance fits one MI355X)", file=sys.stderr)
         return 2
 
     import numpy as np
@@ -237,8 +238,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"{args.workload}: {desc}; a step = {B} instance(s) solved "
                                "batched (FP32 operator/CG, FP64 control) in one engine",
-                   "dims": list(p0.dims) if B == 1 or args.workload != "config3" else "mixed",
-                   "N": p0.N if args.workload != "config3" else int(sum(p.N for p in problems)),
+                   "dims": list(p0.dims) if B == 1 or args.workload != "uneven" else "mixed",
+                   "N": p0.N if args.workload != "uneven" else int(sum(p.N for p in problems)),
                    "batch": B, "lambda": p0.lam, "tol": tol},
         "plain_fp64": {"value": round(plain, 3), "unit": "solves/s",
                        "p50_ms": round(1e3 * statistics.median(lat_p), 3),

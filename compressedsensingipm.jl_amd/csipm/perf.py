@@ -2,7 +2,7 @@
 
     python compressedsensingipm.jl_amd/csipm/perf.py --dims 255 255 255 --dtype f32
     python compressedsensingipm.jl_amd/csipm/perf.py --copy-mib 1024   (copy calibration)
-    python compressedsensingipm.jl_amd/csipm/perf.py --batch 64 --workload config1 --mixed
+    python compressedsensingipm.jl_amd/csipm/perf.py --batch 64 --workload 2d255 --mixed
 
 Prints one JSON line with the per-pass HIP-event times (csipm_kkt_apply_timed),
 the algorithmic bytes of each pass and the achieved fraction of 8 TB/s.  The
@@ -35,7 +35,7 @@ def main():
     ap.add_argument("--mixed", action="store_true", help="FP32 operator of a mixed handle")
     ap.add_argument("--batch", type=int, default=0,
                     help="time the batch engine's CG apply over this many instances instead")
-    ap.add_argument("--workload", default="", help="with --batch: config1 instances "
+    ap.add_argument("--workload", default="", help="with --batch: 2d255 instances "
                     "(else batch_instance problems of --dims)")
     args = ap.parse_args()
 
@@ -104,8 +104,8 @@ def batch_main(args):
     import csipm.workloads as wl
     from csipm.batch import BatchSolver, slots_of
     from csipm.fft_operator import FFTOperator
-    if args.workload == "config1":
-        probs = [wl.config1(instance=i + 1) for i in range(args.batch)]
+    if args.workload == "2d255":
+        probs = [wl.punched_2d_255(instance=i + 1) for i in range(args.batch)]
     else:
         probs = [wl.batch_instance(tuple(args.dims), seed=i + 1) for i in range(args.batch)]
     mixed = args.mixed or args.dtype == "f32"

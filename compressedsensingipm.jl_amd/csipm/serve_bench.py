@@ -6,7 +6,7 @@
 Requests follow the serving workload: Poisson arrivals at `rate` per second;
 70% 1D (N log-uniform [1024, 65536]), 25% 2D (odd side [127, 511]), 5% 3D
 (odd side [63, 191]); each generated from its own seed by the generator of
-configs 0 / 1 / 2 at its shape (generation happens before the clock starts).
+1D / 2D / 3D benchmark generators at its shape (generation happens before the clock starts).
 
 * batched: csipm.batch.serve's loop -- a request's operator handle is made
   by a helper thread when it arrives; it is admitted once that is ready and
@@ -53,11 +53,11 @@ def main():
     from csipm.model import FFTNLPModel, ipm_solve
 
     dev = torch.device("cuda:0")
-    arrivals, shapes, _ = wl.config4_stream(nreq=args.requests, rate=args.rate)
+    arrivals, shapes, _ = wl.stream_requests(nreq=args.requests, rate=args.rate)
     t0 = time.perf_counter()
     probs = []
     for i, s in enumerate(shapes):
-        probs.append(wl.config4_problem(s, seed=1_000_003 * 4 + 1 + i))
+        probs.append(wl.stream_problem(s, seed=1_000_003 * 4 + 1 + i))
         if (i + 1) % 100 == 0:
             print(f"# generated {i + 1}", file=sys.stderr, flush=True)
     print(f"# generated {len(probs)} requests in {time.perf_counter() - t0:.1f} s "

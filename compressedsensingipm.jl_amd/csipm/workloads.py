@@ -5,9 +5,11 @@ Two families:
 * the reference's own test generators, restated:
   - `example_signal` : test/ipm_example_1D.jl:6-9, ipm_example_2D.jl:7, ipm_example_3D.jl:8
   - `centering` / `punching_optimized` : test/punching_centering.jl:74-114, 131-201
-* the benchmark workloads (seeded numpy PCG64 streams, seed = 1_000_003*config
-  + instance; draw order modes -> amplitudes -> phases -> noise -> structured
-  mask -> random dropout): `config0` .. `config4`.
+* the benchmark workloads (seeded numpy PCG64 streams, seed = 1_000_003*family
+  + instance, families 0..4 in the order below; draw order modes ->
+  amplitudes -> phases -> noise -> structured mask -> random dropout):
+  `signal_1d_1024`, `punched_2d_255`, `punched_3d`, `uneven_batch`,
+  `stream_problem` / `stream_requests`.
 
 All arrays are flat vectors in Julia column-major order (dimension 1 fastest);
 `missing` holds 0-based linear indices and z0 is zero there, as
@@ -125,8 +127,8 @@ def reference_example(dims, check=True, seed=1):
 # benchmark workloads
 # ---------------------------------------------------------------------------
 
-def _seed(config, instance):
-    return 1_000_003 * config + instance
+def _seed(family, instance):
+    return 1_000_003 * family + instance
 
 
 def _signal(rng, dims, nmodes, kmax=None):
@@ -190,7 +192,7 @@ def _finish(dims, x, noise_sd, rng, structured, dropout_frac, lam, name):
     return Problem(tuple(dims), y, float(lam), missing.astype(np.int64), name, x)
 
 
-def config0(instance=0):
+def signal_1d_1024(instance=0):
     """1D N=1024, 12 modes, 3 gaps of 32 + 5% dropout, lambda 0.1."""
     rng = np.random.default_rng(_seed(0, instance))
     dims = (1024,)
@@ -202,29 +204,29 @@ def config0(instance=0):
     y = x + noise
     missing = np.unique(np.concatenate([gaps, drop])).astype(np.int64)
     y[missing] = 0.0
-    return Problem(dims, y, 0.1, missing, "config0-1d-1024", x)
+    return Problem(dims, y, 0.1, missing, "1d-1024", x)
 
 
-def config1(instance=0):
+def punched_2d_255(instance=0):
     """2D 255x255, 40 modes, discs r=3 on a spacing-16 lattice, lambda 0.05."""
     rng = np.random.default_rng(_seed(1, instance))
     dims = (255, 255)
     x = _signal(rng, dims, 40)
     return _finish(dims, x, 0.01, rng, _lattice_balls(dims, 16, 3), 0.0, 0.05,
-                   "config1-2d-255x255")
+                   "2d-255x255-punched")
 
 
-def config2(instance=0, side=255):
+def punched_3d(instance=0, side=255):
     """3D side^3, 200 modes, balls r=2 on a spacing-8 lattice + 1% dropout,
     lambda 0.05."""
     rng = np.random.default_rng(_seed(2, instance))
     dims = (side, side, side)
     x = _signal(rng, dims, 200, kmax=min(127, (side - 1) // 2))
     return _finish(dims, x, 0.01, rng, _lattice_balls(dims, 8, 2), 0.01, 0.05,
-                   f"config2-3d-{side}^3")
+                   f"3d-{side}^3-punched")
 
 
-def config3_shapes(n1d=256, n2d=128):
+def uneven_shapes(n1d=256, n2d=128):
     """Shapes of the batched workload: 1D N log-uniform in [512, 65536];
     2D odd sides uniform in [63, 511]."""
     rng = np.random.default_rng(_seed(3, 0))
@@ -262,13 +264,13 @@ def batch_instance(dims, seed, lam=0.1):
     return Problem(dims, y, lam, missing, "x".join(map(str, dims)), x)
 
 
-def config3(n1d=256, n2d=128):
-    shapes = config3_shapes(n1d, n2d)
+def uneven_batch(n1d=256, n2d=128):
+    shapes = uneven_shapes(n1d, n2d)
     return [batch_instance(s, _seed(3, i + 1)) for i, s in enumerate(shapes)]
 
 
-def config4_problem(shape, seed):
-    """One serving request: the generator of configs 0 / 1 / 2 at its shape
+def stream_problem(shape, seed):
+    """One serving request: the 1D / 2D / 3D generator above at its shape
     (1D: 12 + N/128 modes, 3 gaps of 32 + 5% dropout, lambda 0.1; 2D: 40
     modes, r=3 discs on a 16-lattice, lambda 0.05; 3D: 200 modes, r=2 balls
     on an 8-lattice + 1% dropout, lambda 0.05)."""
@@ -287,7 +289,7 @@ def config4_problem(shape, seed):
                    "x".join(map(str, dims)))
 
 
-def config4_stream(nreq=10_000, rate=50.0):
+def stream_requests(nreq=10_000, rate=50.0):
     """Poisson arrivals (exponential gaps) and request shapes of the serving
     workload: 70% 1D (N log-uniform [1024, 65536]), 25% 2D (odd side in
     [127, 511]), 5% 3D (odd side in [63, 191])."""

@@ -22,6 +22,6 @@ def test_workload_table():
     sys.path.insert(0, ROOT)
     import bench
     import csipm.workloads as wl
-    assert set(bench.WORKLOADS) >= {"config1", "config2", "config3"}
-    probs = bench.make_problems(wl, "config1", 2)
+    assert set(bench.WORKLOADS) >= {"1d1024", "2d255", "3d255", "uneven"}
+    probs = bench.make_problems(wl, "2d255", 2)
     assert len(probs) == 2 and probs[0].dims == (255, 255)

@@ -101,7 +101,7 @@ def _support(beta):
 
 
 @pytest.mark.parametrize("mixed", [False, True])
-@pytest.mark.parametrize("case", ["ex1d", "ex2d", "ex3d", "cfg0", "b1d", "b2d", "b3d", "p1d",
+@pytest.mark.parametrize("case", ["ex1d", "ex2d", "ex3d", "s1d", "b1d", "b2d", "b3d", "p1d",
                                   "p2d", "big1d", "bigp1d"])
 def test_solve_vs_oracle(cuda, case, mixed):
     import csipm.workloads as wl
@@ -110,7 +110,7 @@ def test_solve_vs_oracle(cuda, case, mixed):
     p = {"ex1d": lambda: wl.reference_example((128,), False),
          "ex2d": lambda: wl.reference_example((24, 20), False),
          "ex3d": lambda: wl.reference_example((10, 8, 12), False),
-         "cfg0": lambda: wl.config0(),
+         "s1d": lambda: wl.signal_1d_1024(),
          "b1d": lambda: wl.batch_instance((1500,), seed=11),
          "b2d": lambda: wl.batch_instance((63, 63), seed=12),
          "b3d": lambda: wl.batch_instance((26, 21, 15), seed=13),

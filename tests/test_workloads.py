@@ -34,18 +34,18 @@ def test_signal_fft_equals_cosine_sum(dims, nmodes, kmax):
 
 
 def test_config_shapes_and_masks():
-    p = wl.config1()
+    p = wl.punched_2d_255()
     assert p.dims == (255, 255) and p.lam == 0.05
     assert 0.08 < len(p.missing) / p.N < 0.14        # r=3 discs on a 16-lattice
     assert np.all(p.z0[p.missing] == 0.0)
-    shapes = wl.config3_shapes()
+    shapes = wl.uneven_shapes()
     assert len(shapes) == 384
     n1 = [s[0] for s in shapes if len(s) == 1]
     assert min(n1) >= 512 and max(n1) <= 65536
     assert all(s[0] % 2 == 1 and 63 <= s[0] <= 511 for s in shapes if len(s) == 2)
-    arr, sh, lams = wl.config4_stream(nreq=2000)
+    arr, sh, lams = wl.stream_requests(nreq=2000)
     assert np.all(np.diff(arr) >= 0)
     frac = np.array([len(s) for s in sh])
     assert abs((frac == 1).mean() - 0.70) < 0.04 and abs((frac == 3).mean() - 0.05) < 0.02
-    q = wl.config4_problem((127, 127), seed=3)
+    q = wl.stream_problem((127, 127), seed=3)
     assert q.lam == 0.05 and np.all(q.z0[q.missing] == 0.0)
