@@ -1641,6 +1641,7 @@ int csipm_mask_index_missing(const double* weights, int64_t n, int64_t* index_ou
   *count = 0;
   if (n == 0) return CSIPM_OK;
   hipStream_t st = (hipStream_t)stream;
+  std::lock_guard<std::mutex> lk(g_capture_mu);   // allocates and synchronises
   const int64_t per = 256 * (int64_t)kMaskChunks;
   const int64_t nb = (n + per - 1) / per;
   if (nb > INT32_MAX) return fail(CSIPM_ERR_ARG, "mask too large");
