@@ -669,7 +669,30 @@ __device__ __forceinline__ void dim1_body(const PassArgs<T>& a, int tile) {
   const bool even = (N1 & 1) == 0;
   constexpr int K = stage_k<T>();
   const int nl1 = a.nlines - 1;
-  if (MODE != MODE_R2C) {
+  if constexpr (MODE != MODE_R2C) {
+    // the mask words of the tile's lines are loaded with the tile (into LDS)
+    // rather than per element between the transforms: a line's N1 samples
+    // span at most W words
+    constexpr int kMaskWords = 640;
+    constexpr int kMR = (kMaskWords + NT - 1) / NT;
+    __shared__ uint32_t mw[kMR * NT];
+    const int W = (N1 + 31) / 32 + 1;
+    const int nlt = 2 * np;
+    const bool stage_mask = a.miss && nlt * W <= kMaskWords;
+    uint32_t mreg[kMR];
+    if (stage_mask) {
+      const int64_t nwords = ((int64_t)a.nlines * N1 + 31) >> 5;
+#pragma unroll
+      for (int r = 0; r < kMR; ++r) {
+        const int i = threadIdx.x + r * NT;
+        const int ll = i / W, j = i - ll * W;
+        const int64_t l = 2 * (int64_t)p0 + ll;
+        const int64_t w = ((l * N1) >> 5) + j;
+        const bool ok = i < nlt * W && l < a.nlines && w < nwords;
+        const uint32_t v = a.miss[ok ? w : 0];
+        mreg[r] = ok ? v : 0u;
+      }
+    }
     const int tot = H1 << lg;
     staged<NT, K, Pair<T>>(
         tot,
@@ -704,6 +727,10 @@ __device__ __forceinline__ void dim1_body(const PassArgs<T>& a, int tile) {
           lds[k * ps + p] = {A.x - B.y, A.y + B.x};
           if (k > 0 && N1 - k != k) lds[(N1 - k) * ps + p] = {A.x + B.y, B.x - A.y};
         });
+    if (stage_mask) {
+#pragma unroll
+      for (int r = 0; r < kMR; ++r) mw[threadIdx.x + r * NT] = mreg[r];
+    }
     if (dflag) return;
     __syncthreads();
     line_dft<T, E, NT, false>(lds, a, lg, ps);
@@ -717,7 +744,11 @@ __device__ __forceinline__ void dim1_body(const PassArgs<T>& a, int tile) {
           Masked<T> r;
           r.z = lds[n * ps + p];
           r.wa = r.wb = 0u;
-          if (a.miss) {
+          if (stage_mask) {
+            const int64_t sa = (int64_t)la * N1 + n, sb = sa + N1;
+            r.wa = mw[(2 * p) * W + (int)((sa >> 5) - (((int64_t)la * N1) >> 5))];
+            r.wb = mw[(2 * p + 1) * W + (int)((sb >> 5) - (((int64_t)(la + 1) * N1) >> 5))];
+          } else if (a.miss) {
             r.wa = a.miss[((int64_t)min(la, nl1) * N1 + n) >> 5];
             r.wb = a.miss[((int64_t)min(la + 1, nl1) * N1 + n) >> 5];
           }
