@@ -2,9 +2,9 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload 2d255] [--batch B]
 
-Workload 2d255 (default): B instances of the 2D 255x255 generator (40 modes,
-r=3 discs on a 16-lattice, lambda 0.05, tol 1e-8, instance seeds 1..B),
-already resident in HBM.  (1d1024: B 1D N=1024 instances; 3d255: one 3D
+Workload 2d255 (default): B instances (default 256) of the 2D 255x255
+generator (40 modes, r=3 discs on a 16-lattice, lambda 0.05, tol 1e-8,
+instance seeds 1..B), already resident in HBM.  (1d1024: B 1D N=1024 instances; 3d255: one 3D
 255^3 instance; uneven: the 384 uneven 1D/2D instances at tol 1e-6.)
 A step solves all B:
 
@@ -122,7 +122,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="2d255", choices=sorted(WORKLOADS))
-    ap.add_argument("--batch", type=int, default=64)
+    ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
