@@ -29,6 +29,9 @@ SIGNATURES = {
     "csipm_op_destroy": (_i, [_vp]),
     "csipm_op_set_stream": (_i, [_vp, _vp]),
     "csipm_op_info": (_i, [_vp, ctypes.POINTER(_i64), ctypes.POINTER(_i64), ctypes.POINTER(_i)]),
+    "csipm_op_bytes": (_i, [_vp, ctypes.POINTER(_i64)]),
+    "csipm_op_footprint": (_i, [_i, ctypes.POINTER(_i64), _i, _i64, ctypes.POINTER(_i64),
+                                ctypes.POINTER(_i64)]),
     "csipm_beta_to_dft": (_i, [_vp, _vp, _vp, _i]),
     "csipm_dft_to_beta": (_i, [_vp, _vp, _vp, _i]),
     "csipm_m_perp_beta": (_i, [_vp, _vp, _vp]),
@@ -46,7 +49,7 @@ SIGNATURES = {
     "csipm_gondzio_cg_solve": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _d, _i64,
                                     ctypes.POINTER(_i64), ctypes.POINTER(_d),
                                     ctypes.POINTER(_i)]),
-    "csipm_gondzio_solve": (_i, [_vp, _vp, _d, _d, _i64, _d, _i64, _vp, _vp, _vp]),
+    "csipm_gondzio_solve": (_i, [_vp, _vp, _d, _d, _i64, _d, _d, _i64, _vp, _vp, _vp]),
     "csipm_mask_index_missing": (_i, [_vp, _i64, _vp, ctypes.POINTER(_i64), _vp]),
     "csipm_mtm": (_i, [_vp, _vp, _vp]),
     "csipm_kkt_apply": (_i, [_vp, _vp, _vp, _vp, _vp, _d, _d]),
@@ -54,14 +57,14 @@ SIGNATURES = {
     "csipm_precond_apply": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _d, _d]),
     "csipm_cg_solve": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _d, _i64,
                             ctypes.POINTER(_i64), ctypes.POINTER(_d), ctypes.POINTER(_i)]),
-    "csipm_solve": (_i, [_vp, _vp, _d, _d, _i64, _d, _i64, _vp, ctypes.POINTER(_d)]),
+    "csipm_solve": (_i, [_vp, _vp, _d, _d, _i64, _d, _d, _i64, _vp, ctypes.POINTER(_d)]),
     "csipm_admm": (_i, [_vp, _vp, _d, _d, _i64, _d, _vp, ctypes.POINTER(_d)]),
     "csipm_batch_create": (_i, [ctypes.POINTER(_vp), _i, _i64, _i, _vp]),
     "csipm_batch_destroy": (_i, [_vp]),
     "csipm_batch_slots": (_i, [_vp, ctypes.POINTER(_i64)]),
     "csipm_batch_bytes": (_i, [_i, _i64, ctypes.POINTER(_i64)]),
     "csipm_batch_admit": (_i, [_vp, _vp, _vp, _d, _vp, _i64, ctypes.POINTER(_i)]),
-    "csipm_batch_step": (_i, [_vp, _d, _i64, _d, ctypes.POINTER(_i)]),
+    "csipm_batch_step": (_i, [_vp, _d, _i64, _d, _d, ctypes.POINTER(_i)]),
     "csipm_batch_poll": (_i, [_vp, ctypes.POINTER(_i), _i, ctypes.POINTER(_i)]),
     "csipm_batch_harvest": (_i, [_vp, _i, ctypes.POINTER(_d)]),
     "csipm_batch_apply_timed": (_i, [_vp, _i64, ctypes.POINTER(_d), ctypes.POINTER(_d), _i,
@@ -69,7 +72,7 @@ SIGNATURES = {
     "csipm_batch_launch_name": (ctypes.c_char_p, [_vp, _i]),
     "csipm_batch_stats": (_i, [_vp, ctypes.POINTER(_d)]),
     "csipm_batch_solve": (_i, [_vp, _i, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
-                               ctypes.POINTER(_d), _d, _i64, _d, _i64, ctypes.POINTER(_vp),
+                               ctypes.POINTER(_d), _d, _i64, _d, _d, _i64, ctypes.POINTER(_vp),
                                ctypes.POINTER(_d)]),
     "csipm_nlp_obj": (_i, [_vp, _vp, _vp, _d, ctypes.POINTER(_d)]),
     "csipm_nlp_grad": (_i, [_vp, _vp, _vp, _d, _vp]),

@@ -39,6 +39,25 @@ def pool_bytes(slots: int, mixed: bool) -> int:
     return int(b.value)
 
 
+def op_bytes(op: FFTOperator) -> int:
+    """Device bytes the handle holds itself (csipm_op_bytes)."""
+    b = ctypes.c_int64()
+    _lib.call("csipm_op_bytes", op._h, ctypes.byref(b))
+    return int(b.value)
+
+
+def footprint(dims, n_missing, mixed=True):
+    """(pool slots, handle device bytes) of a problem before its handle is
+    made (csipm_op_footprint)."""
+    dims = tuple(int(d) for d in dims)
+    sizes = (ctypes.c_int64 * 3)(*(list(dims) + [1] * (3 - len(dims))))
+    sl = ctypes.c_int64()
+    by = ctypes.c_int64()
+    _lib.call("csipm_op_footprint", len(dims), sizes, _prec(mixed), int(n_missing),
+              ctypes.byref(sl), ctypes.byref(by))
+    return int(sl.value), int(by.value)
+
+
 @dataclass
 class Result:
     beta: torch.Tensor
@@ -51,9 +70,15 @@ class Result:
     tag: object = None
     t_admit: float = 0.0
     t_done: float = 0.0
+    error: str = ""
 
 
 _STATUS = {0: "solved", 1: "max_iter", 2: "failure"}
+
+
+def _rejected(tag, why, t):
+    nan = float("nan")
+    return Result(None, nan, 0, 0, "rejected", nan, nan, tag, t, t, why)
 
 
 class BatchSolver:
@@ -85,7 +110,8 @@ class BatchSolver:
 
     def admit(self, op: FFTOperator, z0, lam, tag=None, cg_itmax=0):
         """Admit a problem; returns its id, or None when the pool or the id
-        table is full (the caller retries after harvesting)."""
+        table is full (the caller retries after harvesting).  A handle serves
+        one live instance at a time: admitting one that is live raises."""
         if op.mixed != self.mixed:
             raise ValueError("operator precision differs from the batch")
         z = op._vec(z0, dtype=torch.float64)
@@ -99,10 +125,13 @@ class BatchSolver:
         self._live[i.value] = (op, z, beta, tag, time.perf_counter())
         return i.value
 
-    def step(self, tol=1e-8, max_iter=200, cg_rtol=1e-10):
+    def step(self, tol=1e-8, max_iter=200, cg_rtol=1e-10, cg_atol=0.0):
+        """One interior-point iteration of every admitted, unfinished
+        instance; returns the number still iterating.  cg_atol / cg_rtol: the
+        FP64 PCG stopping rule (csipm.model.ipm_solve)."""
         r = ctypes.c_int()
-        _lib.call("csipm_batch_step", self._b, float(tol), int(max_iter), float(cg_rtol),
-                  ctypes.byref(r))
+        _lib.call("csipm_batch_step", self._b, float(tol), int(max_iter), float(cg_atol),
+                  float(cg_rtol), ctypes.byref(r))
         return r.value
 
     def finished(self):
@@ -144,7 +173,8 @@ class BatchSolver:
                 for k in range(n.value)]
 
 
-def solve_batch(ops, z0s, lams, mixed=False, tol=1e-8, max_iter=200, cg_rtol=1e-10):
+def solve_batch(ops, z0s, lams, mixed=False, tol=1e-8, max_iter=200, cg_rtol=1e-10,
+                cg_atol=0.0):
     """Solve a fixed set of problems in one batch (all admitted up front).
     Returns the results in input order."""
     slots = sum(slots_of(op) for op in ops)
@@ -156,7 +186,7 @@ def solve_batch(ops, z0s, lams, mixed=False, tol=1e-8, max_iter=200, cg_rtol=1e-
         if i is None:
             raise RuntimeError("batch pool too small")
         ids.append(i)
-    while bs.step(tol, max_iter, cg_rtol) > 0:
+    while bs.step(tol, max_iter, cg_rtol, cg_atol) > 0:
         pass
     out = [None] * len(ops)
     for i in bs.finished():
@@ -173,75 +203,161 @@ class Request:
     problem: object = field(compare=False)
 
 
-def serve(requests, make_op, mixed=True, pool_budget_bytes=200 << 30, max_instances=512,
-          tol=1e-6, max_iter=200, cg_rtol=1e-10, clock=time.perf_counter, on_done=None,
-          prep_workers=4):
+def serve(requests, make_op, mixed=True, budget_bytes=200 << 30, pool_fraction=0.75,
+          max_instances=512, tol=1e-6, max_iter=200, cg_rtol=1e-10, cg_atol=0.0,
+          clock=time.perf_counter, on_done=None, prep_workers=4, scan=64):
     """Continuous batching over an arrival stream.
 
     requests: iterable of Request (arrival time in seconds from the start,
     problem with .dims .z0 .lam .missing); make_op(problem) -> FFTOperator,
-    called by `prep_workers` helper threads as requests arrive (0: inline).
-    A request is admitted once it has arrived, its handle is ready and its
-    slots fit the pool (sized from pool_budget_bytes); every loop turn runs
-    one interior-point iteration of all admitted problems and retires the
-    finished ones.  Returns the list of Results (tag = the request) with
-    admit / finish times relative to the start."""
-    from concurrent.futures import ThreadPoolExecutor
-    slots_budget = max(1, pool_budget_bytes // max(1, pool_bytes(1, mixed)))
+    called by `prep_workers` helper threads (0: inline).
+
+    Device memory: one engine pool of pool_fraction * budget_bytes, made
+    once, plus the handles of the requests being prepared or solved, which
+    share the rest of the budget (csipm_op_footprint before a handle exists,
+    csipm_op_bytes after).  A request whose slots exceed the pool, or whose
+    handle exceeds the handle share, can never run: it comes back at once as
+    a Result with status "rejected" (so does one whose handle cannot be made
+    or admitted).  Handles are made in arrival order while their bytes fit;
+    requests are admitted in arrival order as pool slots and ids free up,
+    and up to `scan` later ready requests may pass one that does not fit.
+    Every loop turn runs one interior-point iteration of all admitted
+    problems and retires the finished ones (their handles are freed at
+    once).  Returns the Results (tag = the request), admit / finish times
+    relative to the start."""
+    from collections import deque
+    from concurrent.futures import FIRST_COMPLETED, ThreadPoolExecutor
+    from concurrent.futures import wait as fwait
+    per_slot = pool_bytes(1, mixed)
+    pool_slots = max(1, int(budget_bytes * pool_fraction) // per_slot)
+    handle_budget = budget_bytes - pool_slots * per_slot
     pending = list(requests)
     heapq.heapify(pending)
-    bs = None
-    waiting = []
+    waiting = deque()   # arrived, not admitted: [req, future or op, bytes held]
     out = []
-    t_start = clock()
-    ops, futs = {}, {}
+    held = 0            # handle bytes of waiting (prepared) and live requests
+    live_bytes = {}     # request seq -> handle bytes
     dev = torch.cuda.current_device()
 
     def prepare(problem):
         with torch.cuda.device(dev):
             return make_op(problem)
 
+    def finish(r):
+        out.append(r)
+        if on_done:
+            on_done(r)
+
     prep = ThreadPoolExecutor(max_workers=prep_workers) if prep_workers > 0 else None
-    while pending or waiting or (bs is not None and bs.live):
-        now = clock() - t_start
-        while pending and pending[0].arrival <= now:
-            req = heapq.heappop(pending)
-            waiting.append(req)
-            if prep is not None:
-                futs[req.seq] = prep.submit(prepare, req.problem)
-        if bs is None:
-            bs = BatchSolver(slots_budget, max_instances=max_instances, mixed=mixed)
-        # admit in arrival order while the pool has room
-        still = []
-        for req in waiting:
-            if req.seq not in ops:
-                f = futs.get(req.seq)
-                if f is not None and not f.done():
-                    still.append(req)
+    bs = BatchSolver(pool_slots, max_instances=max_instances, mixed=mixed)
+    t_start = clock()
+    try:
+        while pending or waiting or bs.live:
+            now = clock() - t_start
+            while pending and pending[0].arrival <= now:
+                req = heapq.heappop(pending)
+                p = req.problem
+                try:
+                    slots, hb = footprint(p.dims, len(p.missing), mixed)
+                except _lib.CsipmError as e:
+                    finish(_rejected(req, str(e), now))
                     continue
-                ops[req.seq] = futs.pop(req.seq).result() if f is not None else \
-                    make_op(req.problem)
-            op = ops[req.seq]
-            i = bs.admit(op, req.problem.z0, req.problem.lam, tag=req)
-            if i is None:
-                still.append(req)
-        waiting = still
-        if bs.live == 0:
-            if pending or waiting:
-                nxt = pending[0].arrival - (clock() - t_start) if pending else 0.0
-                time.sleep(max(0.0002, min(0.001, nxt)))
-            continue
-        bs.step(tol, max_iter, cg_rtol)
-        for i in bs.finished():
-            r = bs.harvest(i)
-            r.t_admit -= t_start
-            r.t_done = clock() - t_start
-            ops.pop(r.tag.seq, None)
-            out.append(r)
-            if on_done:
-                on_done(r)
-    if bs is not None:
+                if slots > pool_slots or hb > handle_budget:
+                    finish(_rejected(req, f"needs {slots} pool slots (pool {pool_slots}) and "
+                                          f"{hb} handle bytes (share {handle_budget})", now))
+                    continue
+                waiting.append([req, None, hb])
+            # make handles, in arrival order, while their bytes fit
+            for w in waiting:
+                if w[1] is not None:
+                    continue
+                if held + w[2] > handle_budget:
+                    break
+                held += w[2]
+                w[1] = prep.submit(prepare, w[0].problem) if prep is not None else \
+                    _made(prepare, w[0].problem)
+            # admit ready requests in arrival order
+            skipped = 0
+            admitted = 0
+            k = 0
+            while k < len(waiting) and skipped <= scan:
+                w = waiting[k]
+                req, f = w[0], w[1]
+                if f is None:
+                    break
+                if not f.done():
+                    skipped += 1
+                    k += 1
+                    continue
+                try:
+                    op = f.result()
+                    nb = op_bytes(op)
+                    held += nb - w[2]
+                    w[2] = nb
+                    i = bs.admit(op, req.problem.z0, req.problem.lam, tag=req)
+                except (_lib.CsipmError, ValueError, RuntimeError) as e:
+                    held -= w[2]
+                    del waiting[k]
+                    finish(_rejected(req, str(e), clock() - t_start))
+                    continue
+                if i is None:
+                    if bs.live == 0:
+                        # an empty engine cannot take it: it never will
+                        held -= w[2]
+                        del waiting[k]
+                        op.close()
+                        finish(_rejected(req, "does not fit an empty engine",
+                                         clock() - t_start))
+                        continue
+                    w[1] = _Ready(op)
+                    skipped += 1
+                    k += 1
+                    continue
+                live_bytes[req.seq] = w[2]
+                del waiting[k]
+                admitted += 1
+            if bs.live == 0:
+                futs = [w[1] for w in waiting if w[1] is not None and not w[1].done()]
+                nxt = pending[0].arrival - (clock() - t_start) if pending else None
+                if futs:
+                    fwait(futs, timeout=nxt if nxt is not None and nxt > 0 else None,
+                          return_when=FIRST_COMPLETED)
+                elif nxt is not None and nxt > 0:
+                    time.sleep(nxt)
+                continue
+            bs.step(tol, max_iter, cg_rtol, cg_atol)
+            for i in bs.finished():
+                r = bs.harvest(i)
+                r.t_admit -= t_start
+                r.t_done = clock() - t_start
+                held -= live_bytes.pop(r.tag.seq, 0)
+                finish(r)
+    finally:
         bs.close()
-    if prep is not None:
-        prep.shutdown()
+        if prep is not None:
+            prep.shutdown(wait=True, cancel_futures=True)
     return out
+
+
+class _Ready:
+    """A made handle (or the error of making it) held in the waiting queue;
+    quacks like a done future."""
+
+    def __init__(self, op, exc=None):
+        self.op = op
+        self.exc = exc
+
+    def done(self):
+        return True
+
+    def result(self):
+        if self.exc is not None:
+            raise self.exc
+        return self.op
+
+
+def _made(fn, *args):
+    try:
+        return _Ready(fn(*args))
+    except (_lib.CsipmError, ValueError, RuntimeError) as e:
+        return _Ready(None, e)

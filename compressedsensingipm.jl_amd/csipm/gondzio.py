@@ -195,9 +195,10 @@ class GondzioStats:
         return self.solution[:n] - self.solution[n:2 * n]
 
 
-def gondzio_solve(nlp: GondzioNLPModel, tol=1e-8, max_iter=200, cg_rtol=1e-10, cg_itmax=0):
+def gondzio_solve(nlp: GondzioNLPModel, tol=1e-8, max_iter=200, cg_rtol=1e-10, cg_itmax=0,
+                  cg_atol=0.0):
     """The library's interior-point driver on the Gondzio problem
-    (csipm_gondzio_solve)."""
+    (csipm_gondzio_solve); cg_atol / cg_rtol as csipm.model.ipm_solve."""
     op = nlp.op_fft
     if op.dtype != torch.float64 and not op.mixed:
         raise NotImplementedError("the driver runs on an FP64 or a mixed-precision operator")
@@ -206,7 +207,7 @@ def gondzio_solve(nlp: GondzioNLPModel, tol=1e-8, max_iter=200, cg_rtol=1e-10, c
     y = torch.empty(op.N, dtype=torch.float64, device=op.device)
     info = (ctypes.c_double * 8)()
     _lib.call("csipm_gondzio_solve", op.handle, _ptr(z0), nlp.lam, float(tol), int(max_iter),
-              float(cg_rtol), int(cg_itmax), _ptr(x), _ptr(y), info)
+              float(cg_atol), float(cg_rtol), int(cg_itmax), _ptr(x), _ptr(y), info)
     status = {0: "solved", 1: "max_iter", 2: "failure"}[int(info[3])]
     return GondzioStats(x, y, info[0], int(info[1]), int(info[2]), status, info[4], info[5],
                         info[6])

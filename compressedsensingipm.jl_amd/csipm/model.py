@@ -219,12 +219,27 @@ class ExecutionStats:
     seconds: float
 
 
-def ipm_solve(nlp: FFTNLPModel, tol=1e-8, max_iter=200, cg_rtol=1e-10, cg_itmax=0):
+# PCG stopping rules of the Newton systems: "relative" stops at
+# sqrt(r'Pr) <= 1e-10 sqrt(r0'P r0); "reference" is the rule the reference
+# passes to Krylov.krylov_solve!, atol = 1e-12, rtol = 0 (src/fft_kkt.jl:339)
+CG_RULES = {"relative": (0.0, 1e-10), "reference": (1e-12, 0.0)}
+
+
+def ipm_solve(nlp: FFTNLPModel, tol=1e-8, max_iter=200, cg_rtol=None, cg_itmax=0,
+              cg_atol=None, cg_rule="relative"):
     """Solve the model with the library's interior-point driver
     (csipm_solve; the reference runs MadNLP, src/CompressedSensingIPM.jl:31-36).
-    Returns the primal beta (device, FP64) and the driver's statistics.  On a
-    mixed-precision model the CG tolerance is floored at 1e-5 (FP32 CG) and
-    every Newton direction gets one FP64 refinement step."""
+    Returns the primal beta (device, FP64) and the driver's statistics.
+
+    The PCG of every Newton system stops at sqrt(r'Pr) <= cg_atol +
+    cg_rtol sqrt(r0'P r0); cg_rule picks the pair ("relative": 0, 1e-10;
+    "reference": 1e-12, 0 as src/fft_kkt.jl:339), explicit cg_atol / cg_rtol
+    override it.  On a mixed-precision model the FP32 CG runs to a relative
+    max(cg_rtol, 1e-5) and every Newton direction gets one FP64 refinement
+    step."""
+    a, r = CG_RULES[cg_rule]
+    cg_atol = a if cg_atol is None else cg_atol
+    cg_rtol = r if cg_rtol is None else cg_rtol
     op = nlp.op_fft
     if op.dtype != torch.float64 and not op.mixed:
         raise NotImplementedError("the driver runs on an FP64 or a mixed-precision operator")
@@ -232,7 +247,7 @@ def ipm_solve(nlp: FFTNLPModel, tol=1e-8, max_iter=200, cg_rtol=1e-10, cg_itmax=
     beta = torch.empty(op.N, dtype=torch.float64, device=op.device)
     info = (ctypes.c_double * 8)()
     _lib.call("csipm_solve", op.handle, _ptr(z0), nlp.lam, float(tol), int(max_iter),
-              float(cg_rtol), int(cg_itmax), _ptr(beta), info)
+              float(cg_atol), float(cg_rtol), int(cg_itmax), _ptr(beta), info)
     status = {0: "solved", 1: "max_iter", 2: "failure"}[int(info[3])]
     return ExecutionStats(beta, info[0], int(info[1]), int(info[2]), status, info[4], info[5],
                           info[6])

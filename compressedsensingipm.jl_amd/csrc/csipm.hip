@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
@@ -50,6 +51,19 @@ static int fail(int code, const char* fmt, ...) {
       return fail(CSIPM_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
                   __FILE__, __LINE__);                                                   \
   } while (0)
+
+// Calls that synchronise or allocate cannot run on a stream that is being
+// captured into a graph (the capture would be invalidated): they return an
+// error instead.
+static int no_capture(hipStream_t st, const char* what) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  HIPCHK(hipStreamIsCapturing(st, &cs));
+  if (cs != hipStreamCaptureStatusNone)
+    return fail(CSIPM_ERR_UNSUPPORTED,
+                "%s synchronises or allocates and cannot run on a stream under graph capture",
+                what);
+  return CSIPM_OK;
+}
 
 // Stream capture of the engine's CG graphs and the device calls of handle
 // creation and destruction (allocations, copies, frees, small kernels)
@@ -140,8 +154,12 @@ struct csipm_op {
   CgState* cg = nullptr;         // device
   CgState* cg_host = nullptr;    // pinned mirror
   void* scratch = nullptr;       // N-vector scratch of the NLP entry points (op precision)
-  void* gz = nullptr;            // 2N-vector scratch of the Gondzio entry points
+  void* gz = nullptr;            // vector scratch of the Gondzio entry points
+  size_t scratch_bytes = 0, gz_bytes = 0;
   Engine* solo = nullptr;        // batch-of-one engine of csipm_solve / csipm_cg_solve
+  // live engine instances of this handle (at most one: an instance's passes
+  // use the handle's scratch spectrum, so two would overwrite each other)
+  std::atomic<int> admitted{0};
   bool has64() const { return prec == CSIPM_F64 || prec == CSIPM_MIXED; }
   bool has32() const { return prec == CSIPM_F32 || prec == CSIPM_MIXED; }
   size_t esize() const { return prec == CSIPM_F64 ? 8 : 4; }   // op / CG precision
@@ -1021,7 +1039,7 @@ int admm_solo(csipm_op* h, const double* z0, double lam, double rho, int64_t max
 // csipm_solve: the interior-point driver on a batch of one
 template <typename TC>
 int solve_solo(csipm_op* h, const double* z0, double lam, double tol, int64_t max_iter,
-               double cg_rtol, int64_t cg_itmax, double* beta_out, double* info,
+               double cg_atol, double cg_rtol, int64_t cg_itmax, double* beta_out, double* info,
                double* c_out = nullptr) {
   const auto t0 = std::chrono::steady_clock::now();
   Engine* E = nullptr;
@@ -1030,7 +1048,7 @@ int solve_solo(csipm_op* h, const double* z0, double lam, double tol, int64_t ma
   CHK(engine_admit(E, h, z0, lam, beta_out, cg_itmax, &id));
   E->inst[(size_t)id].c_out = c_out;
   const int mi = (int)std::min<int64_t>(max_iter, 1 << 30);
-  int rc = engine_run<TC>(E, tol, mi, cg_rtol);
+  int rc = engine_run<TC>(E, tol, mi, cg_atol, cg_rtol);
   if (rc) {
     engine_drop(E, id);
     return rc;
@@ -1080,11 +1098,33 @@ __global__ __launch_bounds__(256) void k_gz_obj_parts(const T* __restrict__ x, i
   store_partials<2>(part, a, ops);
 }
 
+// the handle's Gondzio workspace of at least nvec N-vectors of T, grown on
+// demand and kept (the entry points then allocate nothing per call)
+template <typename T>
+static int gz_workspace(csipm_op* h, int nvec, T** out) {
+  const size_t need = (size_t)nvec * sizeof(T) * (size_t)h->N;
+  if (h->gz_bytes < need) {
+    CHK(no_capture(h->stream, "a Gondzio entry point"));
+    std::lock_guard<std::mutex> lk(g_capture_mu);
+    if (h->gz) {
+      HIPCHK(hipStreamSynchronize(h->stream));
+      HIPCHK(hipFree(h->gz));
+      h->gz = nullptr;
+      h->gz_bytes = 0;
+    }
+    HIPCHK(hipMalloc(&h->gz, need));
+    h->gz_bytes = need;
+  }
+  *out = (T*)h->gz;
+  return CSIPM_OK;
+}
+
 template <typename T>
 static int gz_scratch(csipm_op* h, T** a, T** b) {
-  if (!h->gz) HIPCHK(hipMalloc(&h->gz, 2 * sizeof(T) * (size_t)h->N));
-  *a = (T*)h->gz;
-  *b = (T*)h->gz + h->N;
+  T* w = nullptr;
+  CHK(gz_workspace<T>(h, 3, &w));
+  *a = w;
+  *b = w + h->N;
   return CSIPM_OK;
 }
 
@@ -1182,8 +1222,7 @@ static int gz_kkt_full_t(csipm_op* h, const T* x, T* y, double alpha, double bet
   // s = M (x_q - x_p), formed in y's r block's place only after t is taken
   T* dqp = s;
   CHK(each(h, n, [=] __device__(int64_t i) { dqp[i] = x[n + i] - x[i]; }));
-  T* ms = nullptr;
-  HIPCHK(hipMallocAsync((void**)&ms, sizeof(T) * (size_t)n, h->stream));
+  T* ms = s + n;   // the workspace's third vector
   int rc = op_m<T>(h, dqp, ms);
   const T a = (T)alpha, b = (T)beta;
   if (!rc)
@@ -1201,7 +1240,6 @@ static int gz_kkt_full_t(csipm_op* h, const T* x, T* y, double alpha, double bet
       y[2 * n + i] = v2;
       y[3 * n + i] = v3;
     });
-  (void)hipFreeAsync(ms, h->stream);
   return rc;
 }
 
@@ -1262,7 +1300,7 @@ static int gz_cg_t(csipm_op* h, const T* b, T* x, const T* Wp, const T* Wq, cons
                    int64_t* iters, double* resid, int* status) {
   const int64_t n = h->N;
   T* buf = nullptr;
-  HIPCHK(hipMalloc((void**)&buf, 9 * sizeof(T) * (size_t)n));
+  CHK(gz_workspace<T>(h, 9, &buf));
   T *bt = buf, *yt = buf + 2 * n, *L1 = buf + 4 * n, *L2 = buf + 5 * n, *Q11 = buf + 6 * n,
     *Q12 = buf + 7 * n, *Q22 = buf + 8 * n;
   int rc = each(h, n, [=] __device__(int64_t i) {
@@ -1287,7 +1325,6 @@ static int gz_cg_t(csipm_op* h, const T* b, T* x, const T* Wp, const T* Wq, cons
     hipError_t e = hipStreamSynchronize(h->stream);
     if (e != hipSuccess) rc = fail(CSIPM_ERR_HIP, "gondzio cg: %s", hipGetErrorString(e));
   }
-  (void)hipFree(buf);
   return rc;
 }
 
@@ -1303,6 +1340,10 @@ static void destroy_impl(csipm_op_t h);
 
 int csipm_op_destroy(csipm_op_t h) {
   if (!h) return CSIPM_OK;
+  if (h->admitted.load() > 0)
+    return fail(CSIPM_ERR_ARG, "operator handle is admitted to a live batch instance "
+                "(harvest it or destroy the batch first)");
+  CHK(no_capture(h->stream, "csipm_op_destroy"));
   destroy_impl(h);
   return CSIPM_OK;
 }
@@ -1332,7 +1373,7 @@ constexpr int kBig1D = 4096;
 
 // four-step plan of a long 1D line: F = N when N factors into the supported
 // radices (A x B a balanced divisor pair, both <= 4096), else Bluestein with
-// the power-of-two F >= 2N - 1; twiddles of A, B and F, the chirp tables
+// the power-of-two F >= 2N - 1 (host planning only)
 static int plan_big1d(csipm_op* h) {
   const int N = h->n[0];
   int F = N, A = 0, B = 0;
@@ -1358,6 +1399,12 @@ static int plan_big1d(csipm_op* h) {
   h->bB = B;
   h->bF = F;
   h->plan[0].L = F;
+  return CSIPM_OK;
+}
+
+// device tables of a four-step plan: twiddles of A, B and F, Bluestein tables
+static int upload_big1d(csipm_op* h) {
+  const int N = h->n[0], A = h->bA, B = h->bB, F = h->bF;
   for (PrecState* ps : {&h->p64, &h->p32}) {
     const bool want = ps == &h->p64 ? h->has64() : h->has32();
     if (!want) continue;
@@ -1399,8 +1446,10 @@ static int set_shapes(csipm_op* h, PrecState& ps, size_t cs) {
   return CSIPM_OK;
 }
 
-static int create_impl(csipm_op* h, int dim, const int64_t sizes[3], int rdft, int precision,
-                       const int64_t* missing, int64_t n_missing, void* stream) {
+// host-side planning of a handle: sizes, map geometry, FFT plans, launch
+// shapes and partial-sum capacity.  No device work (csipm_op_footprint runs
+// it alone).
+static int plan_impl(csipm_op* h, int dim, const int64_t sizes[3], int rdft, int precision) {
   if (dim < 1 || dim > 3) return fail(CSIPM_ERR_ARG, "dim must be 1, 2 or 3 (got %d)", dim);
   if (!sizes) return fail(CSIPM_ERR_ARG, "sizes is null");
   // rdft = 0 (the reference's complex path) computes the same operator: the
@@ -1412,7 +1461,6 @@ static int create_impl(csipm_op* h, int dim, const int64_t sizes[3], int rdft, i
     return fail(CSIPM_ERR_ARG, "bad precision %d", precision);
   h->dim = dim;
   h->prec = precision;
-  h->stream = (hipStream_t)stream;
   // 1D lines up to 2^23 (four-step, Bluestein length <= 2^24); 2D / 3D axes up to 16384
   const int64_t smax = dim == 1 ? ((int64_t)1 << 23) : 16384;
   for (int i = 0; i < dim; ++i) {
@@ -1428,9 +1476,51 @@ static int create_impl(csipm_op* h, int dim, const int64_t sizes[3], int rdft, i
   h->Nc = (int64_t)h->H1 * h->n[1] * h->n[2];
   h->U = 2 * h->Nc;
   if (dim == 1 && h->n[0] > kBig1D) CHK(plan_big1d(h));
-  for (int i = 0; i < dim && !h->big; ++i) {
+  for (int i = 0; i < dim && !h->big; ++i)
     if (!plan_axis(h->n[i], h->plan[i], h->bs[i]))
       return fail(CSIPM_ERR_UNSUPPORTED, "no FFT plan for size %d", h->n[i]);
+  if (h->has64()) CHK(set_shapes(h, h->p64, sizeof(cpx<double>)));
+  if (h->has32()) CHK(set_shapes(h, h->p32, sizeof(cpx<float>)));
+  int np = 4096;
+  for (PrecState* ps : {&h->p64, &h->p32}) {
+    np = std::max(np, ps->d1.grid);
+    for (int ax = 1; ax < dim; ++ax) np = std::max(np, ps->lines[ax].grid);
+  }
+  h->npart = 8 * np;   // up to 8 reduced quantities per pass
+  return CSIPM_OK;
+}
+
+// device bytes a handle holds itself: plans, masks, the scratch spectrum,
+// partial sums, Bluestein tables and the lazily made entry-point scratch
+// (twiddle tables are shared between handles and cached: not counted)
+static size_t handle_own_bytes(const csipm_op* h) {
+  size_t b = 5 * sizeof(FftPlan);
+  if (h->nmiss > 0) b += sizeof(uint32_t) * (size_t)((h->N + 31) / 32);
+  b += sizeof(uint32_t) * (size_t)((h->U + 31) / 32);
+  b += (size_t)std::max<int64_t>(h->Nc, h->bF) * sizeof(cpx<double>);
+  b += sizeof(double) * (size_t)h->npart;
+  for (int p = 0; p < 2; ++p) {
+    if (!(p == 0 ? h->has64() : h->has32())) continue;
+    const size_t cs = p == 0 ? sizeof(cpx<double>) : sizeof(cpx<float>);
+    if (h->big) {
+      if (h->obs) b += cs * ((size_t)h->n[0] + 2 * (size_t)h->bF);
+    } else {
+      for (int i = 0; i < h->dim; ++i)
+        if (h->bs[i]) b += cs * ((size_t)h->n[i] + 2 * (size_t)h->plan[i].L);
+    }
+  }
+  return b + h->scratch_bytes + h->gz_bytes;
+}
+
+static int create_impl(csipm_op* h, int dim, const int64_t sizes[3], int rdft, int precision,
+                       const int64_t* missing, int64_t n_missing, void* stream) {
+  CHK(plan_impl(h, dim, sizes, rdft, precision));
+  h->stream = (hipStream_t)stream;
+  CHK(no_capture(h->stream, "csipm_op_create"));
+  if (n_missing < 0) return fail(CSIPM_ERR_ARG, "n_missing < 0");
+  if (n_missing > 0 && !missing) return fail(CSIPM_ERR_ARG, "missing is null");
+  if (h->big) CHK(upload_big1d(h));
+  for (int i = 0; i < dim && !h->big; ++i) {
     const int F = h->plan[i].L;
     if (h->has64()) CHK(upload_twiddles<double>(F, &h->p64.tw[i]));
     if (h->has32()) CHK(upload_twiddles<float>(F, &h->p32.tw[i]));
@@ -1448,11 +1538,7 @@ static int create_impl(csipm_op* h, int dim, const int64_t sizes[3], int rdft, i
     HIPL(hipMalloc(&h->dplan, sizeof(plans)));
     HIPL(hipMemcpy(h->dplan, plans, sizeof(plans), hipMemcpyHostToDevice));
   }
-  if (h->has64()) CHK(set_shapes(h, h->p64, sizeof(cpx<double>)));
-  if (h->has32()) CHK(set_shapes(h, h->p32, sizeof(cpx<float>)));
   // missing-sample bitmask
-  if (n_missing < 0) return fail(CSIPM_ERR_ARG, "n_missing < 0");
-  if (n_missing > 0 && !missing) return fail(CSIPM_ERR_ARG, "missing is null");
   h->nmiss = n_missing;
   if (n_missing > 0) {
     const int64_t words = (h->N + 31) / 32;
@@ -1483,12 +1569,6 @@ static int create_impl(csipm_op* h, int dim, const int64_t sizes[3], int rdft, i
     HIPCHK(hipGetLastError());
   }
   HIPL(hipMalloc(&h->buf, (size_t)std::max<int64_t>(h->Nc, h->bF) * sizeof(cpx<double>)));
-  int np = 4096;
-  for (PrecState* ps : {&h->p64, &h->p32}) {
-    np = std::max(np, ps->d1.grid);
-    for (int ax = 1; ax < dim; ++ax) np = std::max(np, ps->lines[ax].grid);
-  }
-  h->npart = 8 * np;   // up to 8 reduced quantities per pass
   HIPL(hipMalloc(&h->part, sizeof(double) * (size_t)h->npart));
   HIPL(hipStreamSynchronize(h->stream));
   return CSIPM_OK;
@@ -1514,6 +1594,25 @@ int csipm_op_create(csipm_op_t* out, int dim, const int64_t sizes[3], int rdft, 
 int csipm_op_set_stream(csipm_op_t h, void* stream) {
   if (!h) return fail(CSIPM_ERR_ARG, "null handle");
   h->stream = (hipStream_t)stream;
+  return CSIPM_OK;
+}
+
+int csipm_op_bytes(csipm_op_t h, int64_t* bytes) {
+  if (!h || !bytes) return fail(CSIPM_ERR_ARG, "null argument");
+  size_t b = handle_own_bytes(h);
+  if (h->solo) b += engine_bytes_per_slot(h->solo->tc_size) * (size_t)h->solo->cap;
+  *bytes = (int64_t)b;
+  return CSIPM_OK;
+}
+
+int csipm_op_footprint(int dim, const int64_t sizes[3], int precision, int64_t n_missing,
+                       int64_t* slots, int64_t* bytes) {
+  if (!slots || !bytes || n_missing < 0) return fail(CSIPM_ERR_ARG, "bad argument");
+  csipm_op tmp;
+  CHK(plan_impl(&tmp, dim, sizes, 1, precision));
+  tmp.nmiss = n_missing;
+  *slots = slots_of(&tmp);
+  *bytes = (int64_t)handle_own_bytes(&tmp);
   return CSIPM_OK;
 }
 
@@ -1621,6 +1720,48 @@ __global__ __launch_bounds__(256) void k_mask_write(const double* __restrict__ w
   }
 }
 
+// exclusive scan of c[0, nb) in place by one workgroup of 1024 threads, the
+// total into c[nb]: thread t scans a contiguous run of ceil(nb / 1024)
+// entries, the runs' sums are scanned across the workgroup
+__global__ __launch_bounds__(1024) void k_mask_scan(int64_t* __restrict__ c, int64_t nb) {
+  __shared__ int64_t part[1024];
+  const int t = threadIdx.x;
+  const int64_t per = (nb + 1023) / 1024;
+  const int64_t lo0 = (int64_t)t * per;
+  const int64_t lo = lo0 < nb ? lo0 : nb;
+  const int64_t hi = lo + per < nb ? lo + per : nb;
+  int64_t sum = 0;
+  for (int64_t i = lo; i < hi; ++i) sum += c[i];
+  part[t] = sum;
+  __syncthreads();
+  // Hillis-Steele inclusive scan of the run sums
+  for (int d = 1; d < 1024; d <<= 1) {
+    const int64_t v = t >= d ? part[t - d] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  int64_t run = part[t] - sum;   // exclusive prefix of this run
+  for (int64_t i = lo; i < hi; ++i) {
+    const int64_t v = c[i];
+    c[i] = run;
+    run += v;
+  }
+  if (t == 1023) c[nb] = part[1023];
+}
+
+// count, scan and ordered write of the zero entries of w[0, n) into out; ws
+// holds nb + 1 int64 (nb = blocks of 256 * kMaskChunks entries); the total
+// is left in ws[nb].  Asynchronous on st.
+static int mask_index_launch(const double* w, int64_t n, int64_t* out, int64_t* ws, int64_t nb,
+                             hipStream_t st) {
+  hipLaunchKernelGGL(k_mask_count, dim3((unsigned)nb), dim3(256), 0, st, w, n, ws);
+  hipLaunchKernelGGL(k_mask_scan, dim3(1), dim3(1024), 0, st, ws, nb);
+  hipLaunchKernelGGL(k_mask_write, dim3((unsigned)nb), dim3(256), 0, st, w, n, ws, out);
+  HIPCHK(hipGetLastError());
+  return CSIPM_OK;
+}
+
 }  // namespace csipm
 
 int csipm_reconstruct(csipm_op_t h, const void* beta, void* x) {
@@ -1634,6 +1775,25 @@ int csipm_reconstruct(csipm_op_t h, const void* beta, void* x) {
   return rc;
 }
 
+// the count workspace of csipm_mask_index_missing: one per device, grown on
+// demand and kept (calls are serialised by the capture lock)
+static int mask_workspace(int64_t words, int64_t** out) {
+  static std::vector<std::pair<void*, int64_t>> ws;   // by device: (pointer, words)
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if ((int)ws.size() <= dev) ws.resize((size_t)dev + 1, {nullptr, 0});
+  auto& w = ws[(size_t)dev];
+  if (w.second < words) {
+    if (w.first) HIPCHK(hipFree(w.first));
+    w = {nullptr, 0};
+    const int64_t grow = std::max<int64_t>(words, 4096);
+    HIPCHK(hipMalloc(&w.first, sizeof(int64_t) * (size_t)grow));
+    w.second = grow;
+  }
+  *out = (int64_t*)w.first;
+  return CSIPM_OK;
+}
+
 int csipm_mask_index_missing(const double* weights, int64_t n, int64_t* index_out,
                              int64_t* count, void* stream) {
   if (!count || n < 0 || (n > 0 && (!weights || !index_out)))
@@ -1641,35 +1801,17 @@ int csipm_mask_index_missing(const double* weights, int64_t n, int64_t* index_ou
   *count = 0;
   if (n == 0) return CSIPM_OK;
   hipStream_t st = (hipStream_t)stream;
-  std::lock_guard<std::mutex> lk(g_capture_mu);   // allocates and synchronises
+  CHK(no_capture(st, "csipm_mask_index_missing"));
+  std::lock_guard<std::mutex> lk(g_capture_mu);   // may allocate; synchronises
   const int64_t per = 256 * (int64_t)kMaskChunks;
   const int64_t nb = (n + per - 1) / per;
   if (nb > INT32_MAX) return fail(CSIPM_ERR_ARG, "mask too large");
   int64_t* d = nullptr;
-  HIPCHK(hipMalloc((void**)&d, sizeof(int64_t) * (size_t)nb));
-  hipLaunchKernelGGL(k_mask_count, dim3((unsigned)nb), dim3(256), 0, st, weights, n, d);
-  hipError_t e = hipGetLastError();
-  std::vector<int64_t> c((size_t)nb);
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(c.data(), d, sizeof(int64_t) * (size_t)nb, hipMemcpyDeviceToHost, st);
-  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  CHK(mask_workspace(nb + 1, &d));
+  CHK(mask_index_launch(weights, n, index_out, d, nb, st));
   int64_t tot = 0;
-  for (auto& v : c) {
-    const int64_t t = v;
-    v = tot;
-    tot += t;
-  }
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(d, c.data(), sizeof(int64_t) * (size_t)nb, hipMemcpyHostToDevice, st);
-  if (e == hipSuccess) {
-    hipLaunchKernelGGL(k_mask_write, dim3((unsigned)nb), dim3(256), 0, st, weights, n, d,
-                       index_out);
-    e = hipGetLastError();
-  }
-  if (e == hipSuccess) e = hipStreamSynchronize(st);
-  hipFree(d);
-  if (e != hipSuccess)
-    return fail(CSIPM_ERR_HIP, "csipm_mask_index_missing: %s", hipGetErrorString(e));
+  HIPCHK(hipMemcpyAsync(&tot, d + nb, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
   *count = tot;
   return CSIPM_OK;
 }
@@ -1745,14 +1887,19 @@ int csipm_cg_solve(csipm_op_t h, const void* b, void* x, const void* L1, const v
 }
 
 int csipm_solve(csipm_op_t h, const double* z0, double lambda, double tol, int64_t max_iter,
-                double cg_rtol, int64_t cg_itmax, double* beta_out, double* info) {
+                double cg_atol, double cg_rtol, int64_t cg_itmax, double* beta_out,
+                double* info) {
   if (!h || !z0 || !beta_out || !info) return fail(CSIPM_ERR_ARG, "null argument");
   if (!(lambda >= 0) || !(tol > 0) || max_iter < 0)
     return fail(CSIPM_ERR_ARG, "bad lambda / tol / max_iter");
+  if (!(cg_atol >= 0) || !(cg_rtol >= 0) || !(cg_atol > 0 || cg_rtol > 0))
+    return fail(CSIPM_ERR_ARG, "bad cg_atol / cg_rtol");
   if (h->prec == CSIPM_F64)
-    return solve_solo<double>(h, z0, lambda, tol, max_iter, cg_rtol, cg_itmax, beta_out, info);
+    return solve_solo<double>(h, z0, lambda, tol, max_iter, cg_atol, cg_rtol, cg_itmax, beta_out,
+                              info);
   if (h->prec == CSIPM_MIXED)
-    return solve_solo<float>(h, z0, lambda, tol, max_iter, cg_rtol, cg_itmax, beta_out, info);
+    return solve_solo<float>(h, z0, lambda, tol, max_iter, cg_atol, cg_rtol, cg_itmax, beta_out,
+                             info);
   return fail(CSIPM_ERR_UNSUPPORTED, "csipm_solve needs a CSIPM_F64 or CSIPM_MIXED handle");
 }
 
@@ -1771,9 +1918,18 @@ int csipm_admm(csipm_op_t h, const double* z0, double lambda, double rho, int64_
 #define DISPATCH_T(h, F64EXPR, F32EXPR) (OP_DOUBLE(h) ? (F64EXPR) : (F32EXPR))
 
 extern "C++" {
+// N-vector scratch of the NLP entry points, made on first use and kept
+static int nlp_scratch(csipm_op* h) {
+  if (h->scratch) return CSIPM_OK;
+  CHK(no_capture(h->stream, "an NLP-model entry point"));
+  HIPL(hipMalloc(&h->scratch, h->esize() * (size_t)h->N));
+  h->scratch_bytes = h->esize() * (size_t)h->N;
+  return CSIPM_OK;
+}
+
 template <typename T>
 static int nlp_obj_t(csipm_op* h, const T* x, const T* mtz0, double lam, double* f) {
-  if (!h->scratch) HIPCHK(hipMalloc(&h->scratch, h->esize() * (size_t)h->N));
+  CHK(nlp_scratch(h));
   T* y = (T*)h->scratch;
   int rc;
   if (h->dim == 1) rc = apply_m_perp_beta<T, 1>(h, x, y);
@@ -1808,7 +1964,7 @@ static int nlp_grad_t(csipm_op* h, const T* x, const T* mtz0, double lam, T* g) 
 
 template <typename T>
 static int kkt_full_t(csipm_op* h, const T* x, T* y, double alpha, double beta) {
-  if (!h->scratch) HIPCHK(hipMalloc(&h->scratch, h->esize() * (size_t)h->N));
+  CHK(nlp_scratch(h));
   T* Mb = (T*)h->scratch;
   CHK((dispatch_mtm<T, SP_BETA>(h, x, epi_store<T>(Mb), nullptr)));
   hipLaunchKernelGGL((k_kkt_full<T>), dim3(grid_for(h->N, 4)), dim3(kNT), 0, h->stream, x, y, Mb,
@@ -1820,6 +1976,7 @@ static int kkt_full_t(csipm_op* h, const T* x, T* y, double alpha, double beta) 
 
 int csipm_nlp_obj(csipm_op_t h, const void* x, const void* Mtz0, double lambda, double* f) {
   if (!h || !x || !Mtz0 || !f) return fail(CSIPM_ERR_ARG, "null argument");
+  CHK(no_capture(h->stream, "csipm_nlp_obj"));
   return DISPATCH_T(h, nlp_obj_t<double>(h, (const double*)x, (const double*)Mtz0, lambda, f),
                     nlp_obj_t<float>(h, (const float*)x, (const float*)Mtz0, lambda, f));
 }
@@ -1890,6 +2047,7 @@ int csipm_kkt_apply_timed(csipm_op_t h, const void* x, void* y, const void* L1, 
   if (!h || !x || !y || !L1 || !L2 || !pass_ms || !npass) return fail(CSIPM_ERR_ARG, "null argument");
   if (nrep < 1) return fail(CSIPM_ERR_ARG, "nrep < 1");
   if (layout < 0 || layout > 2) return fail(CSIPM_ERR_ARG, "layout must be 0, 1 or 2");
+  CHK(no_capture(h->stream, "csipm_kkt_apply_timed"));
   const char* names[8];
   double bytes[8];
   const int np = kkt_pass_info(h, layout, names, bytes);
@@ -2001,14 +2159,16 @@ int csipm_batch_admit(csipm_batch_t b, csipm_op_t h, const double* z0, double la
   return engine_admit(b->E, h, z0, lambda, beta_out, cg_itmax, id);
 }
 
-int csipm_batch_step(csipm_batch_t b, double tol, int64_t max_iter, double cg_rtol,
-                     int* running) {
+int csipm_batch_step(csipm_batch_t b, double tol, int64_t max_iter, double cg_atol,
+                     double cg_rtol, int* running) {
   if (!b || !running) return fail(CSIPM_ERR_ARG, "null argument");
   if (!(tol > 0) || max_iter < 0) return fail(CSIPM_ERR_ARG, "bad tol / max_iter");
+  if (!(cg_atol >= 0) || !(cg_rtol >= 0) || !(cg_atol > 0 || cg_rtol > 0))
+    return fail(CSIPM_ERR_ARG, "bad cg_atol / cg_rtol");
   const int mi = (int)std::min<int64_t>(max_iter, 1 << 30);
   CHK(engine_enter(b->E));
-  CHK(BATCH_TC(b, engine_step<double>(b->E, tol, mi, cg_rtol, running),
-               engine_step<float>(b->E, tol, mi, cg_rtol, running)));
+  CHK(BATCH_TC(b, engine_step<double>(b->E, tol, mi, cg_atol, cg_rtol, running),
+               engine_step<float>(b->E, tol, mi, cg_atol, cg_rtol, running)));
   return engine_leave(b->E);
 }
 
@@ -2039,8 +2199,8 @@ int csipm_batch_harvest(csipm_batch_t b, int id, double* info) {
 }
 
 int csipm_batch_solve(csipm_batch_t b, int n, const csipm_op_t* ops, const double* const* z0,
-                      const double* lambda, double tol, int64_t max_iter, double cg_rtol,
-                      int64_t cg_itmax, double* const* beta_out, double* info) {
+                      const double* lambda, double tol, int64_t max_iter, double cg_atol,
+                      double cg_rtol, int64_t cg_itmax, double* const* beta_out, double* info) {
   if (!b || n < 0 || (n > 0 && (!ops || !z0 || !lambda || !beta_out || !info)))
     return fail(CSIPM_ERR_ARG, "null argument");
   std::vector<int> ids((size_t)n, -1);
@@ -2053,7 +2213,7 @@ int csipm_batch_solve(csipm_batch_t b, int n, const csipm_op_t* ops, const doubl
   }
   for (;;) {
     int running = 0;
-    const int rc = csipm_batch_step(b, tol, max_iter, cg_rtol, &running);
+    const int rc = csipm_batch_step(b, tol, max_iter, cg_atol, cg_rtol, &running);
     if (rc) {
       for (int i = 0; i < n; ++i) engine_drop(b->E, ids[(size_t)i]);
       return rc;
@@ -2091,6 +2251,7 @@ int csipm_batch_stats(csipm_batch_t b, double* out) {
 
 int csipm_gondzio_obj(csipm_op_t h, const void* x, double lambda, double* f) {
   if (!h || !x || !f) return fail(CSIPM_ERR_ARG, "null argument");
+  CHK(no_capture(h->stream, "csipm_gondzio_obj"));
   return DISPATCH_T(h, gz_obj_t<double>(h, (const double*)x, lambda, f),
                     gz_obj_t<float>(h, (const float*)x, lambda, f));
 }
@@ -2168,11 +2329,13 @@ int csipm_gondzio_cg_solve(csipm_op_t h, const void* b, void* x, const void* Wp,
 }
 
 int csipm_gondzio_solve(csipm_op_t h, const double* z0, double lambda, double tol,
-                        int64_t max_iter, double cg_rtol, int64_t cg_itmax, double* x_out,
-                        double* y_out, double* info) {
+                        int64_t max_iter, double cg_atol, double cg_rtol, int64_t cg_itmax,
+                        double* x_out, double* y_out, double* info) {
   if (!h || !z0 || !x_out || !y_out || !info) return fail(CSIPM_ERR_ARG, "null argument");
   if (!(lambda >= 0) || !(tol > 0) || max_iter < 0)
     return fail(CSIPM_ERR_ARG, "bad lambda / tol / max_iter");
+  if (!(cg_atol >= 0) || !(cg_rtol >= 0) || !(cg_atol > 0 || cg_rtol > 0))
+    return fail(CSIPM_ERR_ARG, "bad cg_atol / cg_rtol");
   if (h->prec != CSIPM_F64 && h->prec != CSIPM_MIXED)
     return fail(CSIPM_ERR_UNSUPPORTED, "csipm_gondzio_solve needs a CSIPM_F64 or CSIPM_MIXED handle");
   const int64_t n = h->N;
@@ -2181,10 +2344,10 @@ int csipm_gondzio_solve(csipm_op_t h, const double* z0, double lambda, double to
   double* beta = x_out + 2 * n;   // r's place holds beta until r is formed
   double* c = y_out;
   const int rc = h->prec == CSIPM_F64
-                     ? solve_solo<double>(h, z0, lambda, tol, max_iter, cg_rtol, cg_itmax, beta,
-                                          info, c)
-                     : solve_solo<float>(h, z0, lambda, tol, max_iter, cg_rtol, cg_itmax, beta,
-                                         info, c);
+                     ? solve_solo<double>(h, z0, lambda, tol, max_iter, cg_atol, cg_rtol,
+                                          cg_itmax, beta, info, c)
+                     : solve_solo<float>(h, z0, lambda, tol, max_iter, cg_atol, cg_rtol,
+                                         cg_itmax, beta, info, c);
   if (rc) return rc;
   const double secs = info[6];
   CHK(each(h, n, [=] __device__(int64_t i) {

@@ -275,6 +275,8 @@ struct Engine {
     graph_version = -1;
   }
   ~Engine() {
+    for (Inst& in : inst)
+      if (in.h) in.h->admitted.fetch_sub(1);
     release_graphs();
     release_tables();
     if (stream) (void)hipStreamDestroy(stream);
@@ -343,6 +345,7 @@ int engine_create(Engine** out, int64_t cap_slots, int maxinst, int tc_size, boo
 
 // order the engine's stream after the caller's work, and the caller's after ours
 int engine_enter(Engine* E) {
+  CHK(no_capture(E->user, "a batch-engine call"));
   HIPCHK(hipEventRecord(E->ev, E->user));
   HIPCHK(hipStreamWaitEvent(E->stream, E->ev, 0));
   return CSIPM_OK;
@@ -676,6 +679,11 @@ int engine_admit(Engine* E, csipm_op* h, const double* z0, double lam, double* b
       break;
     }
   if (k < 0) return fail(CSIPM_ERR_ALLOC, "engine full (%d instances)", E->maxinst);
+  // an instance's passes run in its handle's scratch spectrum: a handle
+  // serves one live instance at a time (in any engine)
+  if (h->admitted.load() > 0)
+    return fail(CSIPM_ERR_ARG, "operator handle is already admitted to a live instance "
+                "(one handle per instance: its scratch spectrum is per handle)");
   const int64_t len = slots_of(h);
   // partial-sum capacity: 8 quantities per vector block, the CG apply's final pass
   const int64_t pcap = len / kSlotAlign * kPartPerAlign;
@@ -714,12 +722,14 @@ int engine_admit(Engine* E, csipm_op* h, const double* z0, double lam, double* b
   HIPCHK(hipStreamSynchronize(E->stream));
   E->active.push_back(k);
   E->tables_dirty = true;
+  h->admitted.fetch_add(1);
   *id = k;
   return CSIPM_OK;
 }
 
 void engine_drop(Engine* E, int id) {
   Engine::Inst& in = E->inst[(size_t)id];
+  if (in.h) in.h->admitted.fetch_sub(1);
   slots_free(E, in.off, in.len);
   in = Engine::Inst{};
   E->active.erase(std::remove(E->active.begin(), E->active.end(), id), E->active.end());
@@ -901,8 +911,8 @@ int engine_cg(Engine* E, int nb, double atol, double rtol, int reliable) {
 constexpr double kRefineRtol = 1e-2;
 
 template <typename TC>
-int engine_newton(Engine* E, double rtol, int reliable, bool refine) {
-  if (!E->mixed) return engine_cg<TC>(E, 2, 0.0, rtol, 0);
+int engine_newton(Engine* E, double atol, double rtol, int reliable, bool refine) {
+  if (!E->mixed) return engine_cg<TC>(E, 2, atol, rtol, 0);
   hipStream_t st = E->stream;
   const BSeg& sg = E->vseg;
   CHK(engine_cg<TC>(E, 1, 0.0, rtol, reliable));
@@ -931,7 +941,8 @@ int engine_newton(Engine* E, double rtol, int reliable, bool refine) {
 // corrector step.  *running = number of instances still iterating.
 // ---------------------------------------------------------------------------
 template <typename TC>
-int engine_step(Engine* E, double tol, int max_iter, double cg_rtol, int* running) {
+int engine_step(Engine* E, double tol, int max_iter, double cg_atol, double cg_rtol,
+                int* running) {
   StatTimer tm(&E->stats[Engine::ST_STEP]);
   E->stats[Engine::ST_NSTEP] += 1;
   const int nb = E->mixed ? 1 : 2;
@@ -944,7 +955,12 @@ int engine_step(Engine* E, double tol, int max_iter, double cg_rtol, int* runnin
   const BSeg& sg = E->vseg;
   const int g = E->vgrid;
   const double tau = 0.995;
+  // FP64: the caller's PCG rule sqrt(r'Pr) <= cg_atol + cg_rtol sqrt(r0'P r0)
+  // (the reference passes atol = 1e-12, rtol = 0).  Mixed: the FP32 CG stops
+  // at a relative 1e-5 at the tightest (an absolute FP64-level target is out
+  // of FP32's reach); accuracy comes from the FP64 refinement
   const double rtol = E->mixed ? std::max(cg_rtol, 1e-5) : cg_rtol;
+  const double atol = E->mixed ? 0.0 : cg_atol;
   const int reliable = E->mixed ? 6 * kCgChunk : 0;
   double *b = E->W(V_B), *beta = E->W(V_BETA), *c = E->W(V_C), *z1 = E->W(V_Z1);
   double *z2 = E->W(V_Z2), *Qb = E->W(V_QB), *s1 = E->W(V_S1), *s2 = E->W(V_S2);
@@ -972,7 +988,7 @@ int engine_step(Engine* E, double tol, int max_iter, double cg_rtol, int* runnin
                        E->C<TC>(C_P11), E->C<TC>(C_P12), E->C<TC>(C_P22), rhs, E->valid);
   HIPCHK(hipGetLastError());
   // predictor
-  CHK(engine_newton<TC>(E, rtol, reliable, false));
+  CHK(engine_newton<TC>(E, atol, rtol, reliable, false));
   auto dir = [&](int corr) -> int {
     if (E->mixed)
       hipLaunchKernelGGL((bk_ipm_dir<double>), dim3(g), dim3(kNT), 0, st, sg, E->info, E->flags,
@@ -995,7 +1011,7 @@ int engine_step(Engine* E, double tol, int max_iter, double cg_rtol, int* runnin
                      beta, c, z1, z2, Qb, b, dt1, dt2, dz1, dz2, E->scal, rhs, rhsd, E->W(V_L1D),
                      E->W(V_L2D), E->valid);
   HIPCHK(hipGetLastError());
-  CHK(engine_newton<TC>(E, rtol, reliable, true));
+  CHK(engine_newton<TC>(E, atol, rtol, reliable, true));
   CHK(dir(1));
   CHK(finalize(E, sg, E->ids, na, {2, 2}, 3, 0, tau, tol, max_iter));
   if (E->mixed)
@@ -1034,10 +1050,10 @@ int engine_harvest(Engine* E, int id, double* info) {
 }
 
 template <typename TC>
-int engine_run(Engine* E, double tol, int max_iter, double cg_rtol) {
+int engine_run(Engine* E, double tol, int max_iter, double cg_atol, double cg_rtol) {
   for (;;) {
     int running = 0;
-    CHK(engine_step<TC>(E, tol, max_iter, cg_rtol, &running));
+    CHK(engine_step<TC>(E, tol, max_iter, cg_atol, cg_rtol, &running));
     if (running == 0) return CSIPM_OK;
   }
 }

@@ -17,6 +17,11 @@
  *
  * Status: 0 on success; otherwise a CSIPM_ERR_* code with a message available
  * from csipm_last_error() (thread-local).
+ *
+ * Entry points that synchronise or allocate (handle creation/destruction,
+ * the solves, the batch calls, csipm_mask_index_missing, the synchronous
+ * objective evaluations) return CSIPM_ERR_UNSUPPORTED when their stream is
+ * being captured into a HIP graph, instead of invalidating the capture.
  */
 #ifndef CSIPM_H
 #define CSIPM_H
@@ -79,6 +84,21 @@ int csipm_op_destroy(csipm_op_t h);
  * after a copy to the host.                                               */
 int csipm_mask_index_missing(const double* weights, int64_t n, int64_t* index_out,
                              int64_t* count, void* stream);
+
+/* Device bytes handle h holds itself: plans, sampling and slot masks, the
+ * scratch spectrum, partial sums, Bluestein tables, lazily made entry-point
+ * scratch and its batch-of-one engine (csipm_solve / csipm_cg_solve); the
+ * FFT twiddle tables are shared between handles and cached, not counted.
+ * FFTOperator's buffers (src/fft_utils.jl:16-54) are what a Julia caller
+ * would count; the serving loop budgets device memory with it.            */
+int csipm_op_bytes(csipm_op_t h, int64_t* bytes);
+/* Footprint of a handle before it is made: the batch-pool slots an instance
+ * of it occupies (csipm_batch_slots) and csipm_op_bytes of the new handle
+ * with n_missing missing samples, from the host-side planning alone (no
+ * device work).  Lets an admission loop reject a request that can never
+ * fit its budget before building anything.                                */
+int csipm_op_footprint(int dim, const int64_t sizes[3], int precision, int64_t n_missing,
+                       int64_t* slots, int64_t* bytes);
 
 /* Change the stream later calls on h are issued to. */
 int csipm_op_set_stream(csipm_op_t h, void* stream);
@@ -180,14 +200,20 @@ const char* csipm_kkt_pass_name(csipm_op_t h, int layout, int pass);
  * MadNLP (ipm_solve!, src/CompressedSensingIPM.jl:31-36).
  *   z0        device, N doubles (observed samples, zero at missing)
  *   lambda    l1 weight;  tol  IPM tolerance;  max_iter  IPM iteration cap
- *   cg_rtol   relative PCG tolerance per Newton system; cg_itmax cap
+ *   cg_atol, cg_rtol  PCG stopping rule of every Newton system (FP64):
+ *             sqrt(r'Pr) <= cg_atol + cg_rtol*sqrt(r0'P r0).  The reference
+ *             passes cg_atol = 1e-12, cg_rtol = 0 to Krylov.krylov_solve!
+ *             (src/fft_kkt.jl:339).  CSIPM_MIXED handles run their FP32 CG
+ *             to max(cg_rtol, 1e-5) relative and ignore cg_atol (accuracy
+ *             comes from the FP64 refinement step).  At least one > 0.
+ *   cg_itmax  CG iteration cap per Newton system (<= 0: 4N)
  *   beta_out  device, N doubles: the recovered beta
  *   info      host array of 8 doubles: [objective, ipm iterations, total cg
  *             iterations, status (0 solved, 1 max_iter, 2 failure),
  *             final dual residual, final mu, seconds, reserved]
  * Synchronous.                                                           */
 int csipm_solve(csipm_op_t h, const double* z0, double lambda, double tol,
-                int64_t max_iter, double cg_rtol, int64_t cg_itmax,
+                int64_t max_iter, double cg_atol, double cg_rtol, int64_t cg_itmax,
                 double* beta_out, double* info);
 
 /* ADMM for the same problem (CompressedSensingADMM, src/admm.jl:8-61):
@@ -258,8 +284,8 @@ int csipm_gondzio_cg_solve(csipm_op_t h, const void* b, void* x, const void* Wp,
  *   info   as csipm_solve, info[0] = 1/2 r'r + lambda sum(p + q).
  * Handle precision CSIPM_F64 or CSIPM_MIXED.  Synchronous.               */
 int csipm_gondzio_solve(csipm_op_t h, const double* z0, double lambda, double tol,
-                        int64_t max_iter, double cg_rtol, int64_t cg_itmax, double* x_out,
-                        double* y_out, double* info);
+                        int64_t max_iter, double cg_atol, double cg_rtol, int64_t cg_itmax,
+                        double* x_out, double* y_out, double* info);
 
 /* ------------------------------------------------------------------------
  * Batch engine: many solves of uneven sizes and dimensionalities in the same
@@ -287,14 +313,18 @@ int csipm_batch_slots(csipm_op_t h, int64_t* slots);
 int csipm_batch_bytes(int precision, int64_t slots, int64_t* bytes);
 /* Admit a problem: z0 (device, N doubles) and beta_out (device, N doubles)
  * stay owned by the caller and must live until the instance is harvested.
- * CSIPM_ERR_ALLOC when the pool has no free range or all ids are taken.  */
+ * CSIPM_ERR_ALLOC when the pool has no free range or all ids are taken.
+ * A handle serves one live instance at a time, in any batch (an instance's
+ * passes run in its handle's scratch spectrum): admitting a handle that is
+ * already live fails with CSIPM_ERR_ARG, and so does csipm_op_destroy of
+ * an admitted handle.                                                     */
 int csipm_batch_admit(csipm_batch_t b, csipm_op_t h, const double* z0,
                       double lambda, double* beta_out, int64_t cg_itmax, int* id);
 /* One interior-point iteration of every admitted, unfinished instance
  * (newly admitted ones are initialised first).  *running: instances still
  * iterating.  Finished ones are listed by csipm_batch_poll.              */
 int csipm_batch_step(csipm_batch_t b, double tol, int64_t max_iter,
-                     double cg_rtol, int* running);
+                     double cg_atol, double cg_rtol, int* running);
 int csipm_batch_poll(csipm_batch_t b, int* ids, int cap, int* n);
 /* Write a finished instance's beta into its beta_out, fill info[8] as
  * csipm_solve does (seconds = 0) and free its slots and id.              */
@@ -303,7 +333,7 @@ int csipm_batch_harvest(csipm_batch_t b, int id, double* info);
  * info: n * 8 doubles.                                                   */
 int csipm_batch_solve(csipm_batch_t b, int n, const csipm_op_t* ops,
                       const double* const* z0, const double* lambda, double tol,
-                      int64_t max_iter, double cg_rtol, int64_t cg_itmax,
+                      int64_t max_iter, double cg_atol, double cg_rtol, int64_t cg_itmax,
                       double* const* beta_out, double* info);
 /* Timing of the batch's CG operator apply (one launch per pass and
  * dimensionality over all admitted instances, vectors zeroed): HIP events

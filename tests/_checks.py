@@ -1,0 +1,58 @@
+"""Shared checks of the parity tests (not a test module).
+
+support_flips / assert_same_support: the support rule of the solves -- the
+same support above rel * max|ref| (rel = 1e-6).  A flip is an index where
+exactly one of the two solutions exceeds the threshold; the assertion lists
+every flip with both values in units of the threshold instead of exempting
+any band around it.
+
+certificate: the algorithm-free LASSO optimality certificate of oracle.ipm
+(prox-gradient fixed-point residual and duality gap of the residual-scaled
+dual point, P(beta) - P* <= gap), computed with the CPU oracle operator.
+"""
+import numpy as np
+
+from oracle import ipm
+from oracle import operator as op
+
+
+def support_flips(beta, ref, rel=1e-6):
+    beta = np.asarray(beta)
+    ref = np.asarray(ref)
+    thr = rel * float(np.abs(ref).max())
+    idx = np.nonzero((np.abs(beta) > thr) != (np.abs(ref) > thr))[0]
+    return idx, thr
+
+
+def assert_same_support(beta, ref, name, rel=1e-6):
+    idx, thr = support_flips(beta, ref, rel)
+    if len(idx):
+        det = [(int(i), round(float(beta[i]) / thr, 4), round(float(ref[i]) / thr, 4))
+               for i in idx[:12]]
+        raise AssertionError(f"{name}: {len(idx)} support flips above {rel:g}*max|ref| "
+                             f"(index, beta/thr, ref/thr): {det}")
+    return int(np.sum(np.abs(np.asarray(ref)) > thr))
+
+
+def certificate(p, beta, workers=8):
+    """(fixed-point residual, duality gap, primal value) of beta for problem p."""
+    b = op.m_perpt_z(p.z0, p.dims, workers=workers)
+    mt = lambda v: op.mtm(v, p.dims, p.missing, workers=workers)            # noqa: E731
+    mp = lambda v: op.m_perp_beta(v, p.dims, p.missing, workers=workers)    # noqa: E731
+    return ipm.lasso_certificate(np.asarray(beta), b, p.lam, mt, mp, p.z0)
+
+
+def oracle_solve(p, tol, workers=8, cg_rtol=None):
+    """Oracle FP64 solve and its objective f(beta, |beta|)."""
+    b = op.m_perpt_z(p.z0, p.dims, workers=workers)
+    mt = lambda v: op.mtm(v, p.dims, p.missing, workers=workers)            # noqa: E731
+    opts = ipm.IPMOptions(tol=tol) if cg_rtol is None else ipm.IPMOptions(tol=tol, cg_rtol=cg_rtol)
+    r = ipm.solve(b, p.lam, mt, opts)
+    return r, objective(p, r.beta, workers)
+
+
+def objective(p, beta, workers=8):
+    beta = np.asarray(beta)
+    b = op.m_perpt_z(p.z0, p.dims, workers=workers)
+    return ipm.objective(beta, np.abs(beta), op.mtm(beta, p.dims, p.missing, workers=workers),
+                         b, p.lam)

@@ -3,8 +3,13 @@
 * A batch of uneven 1D / 2D / 3D problems gives, instance by instance, the
   result of the same problem solved alone (csipm_solve, a batch of one):
   objective within 1e-9 relative, beta within 1e-6 relative l2, the same
-  support above 1e-6 max|beta| on entries clearly away from the threshold.  (A batch tiles its
+  support above 1e-6 max|beta| (every flip listed).  (A batch tiles its
   FFT passes differently, so the CG dot products round differently.)
+* Admission rules: a handle serves one live instance (admitting it twice
+  raises); serve() returns a request that can never fit its budget as
+  "rejected" at once and solves the others.
+* Failure statuses forced from the host: an IPM iteration cap, a CG
+  iteration cap of one, a non-finite sample in z0.
 * Continuous batching: problems admitted while others are mid-solve, and
   retired as they finish, still give the alone results.
 * The mixed-precision batch reaches the FP64 oracle objective within 1e-6.
@@ -16,6 +21,7 @@ import numpy as np
 import pytest
 import torch
 
+import _checks as chk
 from oracle import ipm
 from oracle import operator as op
 
@@ -46,10 +52,7 @@ def _same(r, a, name):
     x = r.beta.cpu().numpy()
     y = a.solution.cpu().numpy()
     assert np.linalg.norm(x - y) <= 1e-6 * np.linalg.norm(y), name
-    thr = 1e-6 * np.abs(y).max()
-    clear = ((np.abs(y) > 10 * thr) | (np.abs(y) < 0.1 * thr)) & \
-        ((np.abs(x) > 10 * thr) | (np.abs(x) < 0.1 * thr))
-    assert np.array_equal((np.abs(x) > thr)[clear], (np.abs(y) > thr)[clear]), name
+    chk.assert_same_support(x, y, name)
 
 
 def _alone(p, mixed, tol):
@@ -63,7 +66,7 @@ def _alone(p, mixed, tol):
 def test_batch_equals_alone(cuda, mixed):
     from csipm.batch import solve_batch
     print("batch vs alone", "mixed" if mixed else "f64", flush=True)
-    tol = 1e-7 if mixed else 1e-8
+    tol = 1e-8
     probs = _problems()
     ops = [_op(p, mixed) for p in probs]
     res = solve_batch(ops, [p.z0 for p in probs], [p.lam for p in probs], mixed=mixed, tol=tol)
@@ -106,7 +109,7 @@ def test_mixed_batch_vs_oracle(cuda):
     print("mixed batch vs oracle", flush=True)
     probs = _problems()
     ops = [_op(p, True) for p in probs]
-    res = solve_batch(ops, [p.z0 for p in probs], [p.lam for p in probs], mixed=True, tol=1e-7)
+    res = solve_batch(ops, [p.z0 for p in probs], [p.lam for p in probs], mixed=True, tol=1e-8)
     for p, r in zip(probs, res):
         b = op.m_perpt_z(p.z0, p.dims)
         mt = lambda v: op.mtm(v, p.dims, p.missing)    # noqa: E731
@@ -174,9 +177,89 @@ def test_serve_stream(cuda):
     print("serve stream", flush=True)
     probs = _problems()[:6]
     reqs = [Request(0.002 * k, k, p) for k, p in enumerate(probs)]
-    out = serve(reqs, lambda p: _op(p, True), mixed=True, pool_budget_bytes=1 << 30,
+    out = serve(reqs, lambda p: _op(p, True), mixed=True, budget_bytes=1 << 30,
                 tol=1e-7, prep_workers=2)
     assert sorted(r.tag.seq for r in out) == list(range(len(probs)))
     for r in out:
         p = probs[r.tag.seq]
         _same(r, _alone(p, True, 1e-7), p.name)
+
+
+def test_handle_admitted_once(cuda):
+    """One handle per live instance: its passes run in the handle's scratch
+    spectrum, so a second admission of a live handle is refused (and so is
+    destroying it); after harvest it can be admitted again."""
+    import csipm.workloads as wl
+    from csipm import _lib
+    from csipm.batch import BatchSolver, slots_of
+    print("handle admitted once", flush=True)
+    p = wl.batch_instance((63, 63), seed=51)
+    o = _op(p, True)
+    bs = BatchSolver(4 * slots_of(o), max_instances=4, mixed=True)
+    assert bs.admit(o, p.z0, p.lam, tag=0) is not None
+    with pytest.raises(_lib.CsipmError) as e:
+        bs.admit(o, p.z0, p.lam, tag=1)
+    assert e.value.code == _lib.ERR_ARG
+    assert _lib.lib().csipm_op_destroy(o._h) == _lib.ERR_ARG
+    while bs.step(tol=1e-7) > 0:
+        pass
+    r = bs.harvest(bs.finished()[0])
+    assert r.status == "solved"
+    assert bs.admit(o, p.z0, p.lam, tag=2) is not None
+    bs.close()          # drops the live instance: the handle is free again
+    o.close()
+
+
+def test_serve_rejects_oversize(cuda):
+    """A request whose pool slots exceed the serving budget comes back at
+    once as "rejected" (a loop that kept it waiting never ended); the other
+    requests are still solved, each as alone."""
+    import csipm.workloads as wl
+    from csipm.batch import Request, footprint, pool_bytes, serve
+    print("serve rejects an oversize request", flush=True)
+    small = [wl.batch_instance((63, 63), seed=61), wl.batch_instance((1500,), seed=62),
+             wl.batch_instance((61, 53), seed=63)]
+    big = wl.batch_instance((40, 40, 40), seed=64)
+    slots_big, _ = footprint(big.dims, len(big.missing), True)
+    slots_small = max(footprint(p.dims, len(p.missing), True)[0] for p in small)
+    assert slots_big > 3 * slots_small
+    # a pool that holds two small requests but never the big one
+    budget = int(pool_bytes(2 * slots_small, True) / 0.75) + (64 << 20)
+    probs = small[:2] + [big] + small[2:]
+    reqs = [Request(0.001 * k, k, p) for k, p in enumerate(probs)]
+    t0 = __import__("time").perf_counter()
+    out = serve(reqs, lambda p: _op(p, True), mixed=True, budget_bytes=budget, tol=1e-7,
+                prep_workers=2)
+    assert __import__("time").perf_counter() - t0 < 60
+    got = {r.tag.seq: r for r in out}
+    assert sorted(got) == list(range(len(probs)))
+    assert got[2].status == "rejected" and "pool slots" in got[2].error
+    for k in (0, 1, 3):
+        _same(got[k], _alone(probs[k], True, 1e-7), probs[k].name)
+
+
+def test_failure_statuses(cuda):
+    """Statuses of runs stopped early, forced from the host: an IPM
+    iteration cap gives "max_iter"; a CG cap of one iteration per Newton
+    system still ends (the reference ignores the CG status, fft_kkt.jl:353)
+    with a finite objective; a NaN sample in z0 gives "failure" after the
+    first residual check instead of iterating to the cap."""
+    import csipm.workloads as wl
+    from csipm.fft_operator import FFTParameters
+    from csipm.model import FFTNLPModel, ipm_solve
+    print("failure statuses", flush=True)
+    p = wl.batch_instance((63, 63), seed=71)
+    for mixed in (False, True):
+        nlp = FFTNLPModel(FFTParameters(2, p.dims, p.z0, p.lam, p.missing), mixed=mixed)
+        st = ipm_solve(nlp, tol=1e-8, max_iter=2)
+        assert st.status == "max_iter" and st.iterations == 2, (mixed, st.status)
+        st = ipm_solve(nlp, tol=1e-8, max_iter=40, cg_itmax=1)
+        assert st.status in ("max_iter", "solved", "failure") and st.iterations <= 40, mixed
+        # at most one CG iteration per Newton system (predictor, corrector and,
+        # mixed, the refinement correction)
+        assert st.cg_iterations <= 3 * (st.iterations + 1), (mixed, st.cg_iterations)
+        z = p.z0.copy()
+        z[5] = np.nan
+        bad = FFTNLPModel(FFTParameters(2, p.dims, z, p.lam, p.missing), mixed=mixed)
+        st = ipm_solve(bad, tol=1e-8, max_iter=50)
+        assert st.status == "failure" and st.iterations <= 1, (mixed, st.status, st.iterations)
