@@ -51,6 +51,7 @@ SIGNATURES = {
                                     ctypes.POINTER(_i)]),
     "csipm_gondzio_solve": (_i, [_vp, _vp, _d, _d, _i64, _d, _d, _i64, _vp, _vp, _vp]),
     "csipm_mask_index_missing": (_i, [_vp, _i64, _vp, ctypes.POINTER(_i64), _vp]),
+    "csipm_mask_index_missing_noalloc": (_i, [_vp, _i64, _vp, ctypes.POINTER(_i64), _vp]),
     "csipm_mtm": (_i, [_vp, _vp, _vp]),
     "csipm_kkt_apply": (_i, [_vp, _vp, _vp, _vp, _vp, _d, _d]),
     "csipm_precond_build": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),

@@ -1680,6 +1680,8 @@ namespace csipm {
 // (wave ballots for the order inside a chunk of 256 entries).
 // ---------------------------------------------------------------------------
 constexpr int kMaskChunks = 16;   // chunks of 256 entries per block
+static_assert(CSIPM_MASK_NOALLOC_MAX == (int64_t)(CSIPM_MASK_NOALLOC_WORDS - 1) * 256 * kMaskChunks,
+              "csipm.h's sample limit of the no-alloc mask entry point");
 
 __global__ __launch_bounds__(256) void k_mask_count(const double* __restrict__ w, int64_t n,
                                                     int64_t* __restrict__ counts) {
@@ -1808,6 +1810,58 @@ int csipm_mask_index_missing(const double* weights, int64_t n, int64_t* index_ou
   if (nb > INT32_MAX) return fail(CSIPM_ERR_ARG, "mask too large");
   int64_t* d = nullptr;
   CHK(mask_workspace(nb + 1, &d));
+  CHK(mask_index_launch(weights, n, index_out, d, nb, st));
+  int64_t tot = 0;
+  HIPCHK(hipMemcpyAsync(&tot, d + nb, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  *count = tot;
+  return CSIPM_OK;
+}
+
+// The count workspace of csipm_mask_index_missing_noalloc: a fixed array in
+// each device's global memory, held until the process ends.  Host code never
+// passes the array's name to a kernel (on the host it is only a stand-in for
+// the symbol): the device address of this device's copy is looked up once per
+// device with hipGetSymbolAddress and kept in g_mask_scratch_ptr.
+static __device__ __align__(256) int64_t scratch[CSIPM_MASK_NOALLOC_WORDS];
+
+static std::mutex g_mask_scratch_mu;
+static std::vector<int64_t*> g_mask_scratch_ptr;   // by device; nullptr until looked up
+
+static int mask_scratch_ptr(int64_t** out) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_mask_scratch_mu);
+  if ((int)g_mask_scratch_ptr.size() <= dev) g_mask_scratch_ptr.resize((size_t)dev + 1, nullptr);
+  int64_t*& p = g_mask_scratch_ptr[(size_t)dev];
+  if (!p) {
+    void* a = nullptr;
+    HIPCHK(hipGetSymbolAddress(&a, HIP_SYMBOL(scratch)));
+    if (!a) return fail(CSIPM_ERR_HIP, "no device address for the mask scratch array");
+    p = (int64_t*)a;
+  }
+  *out = p;
+  return CSIPM_OK;
+}
+
+int csipm_mask_index_missing_noalloc(const double* weights, int64_t n, int64_t* index_out,
+                                     int64_t* count, void* stream) {
+  if (!count || n < 0 || (n > 0 && (!weights || !index_out)))
+    return fail(CSIPM_ERR_ARG, "null argument or negative size");
+  *count = 0;
+  const int64_t per = 256 * (int64_t)kMaskChunks;
+  if (n > CSIPM_MASK_NOALLOC_MAX)
+    return fail(CSIPM_ERR_ARG,
+                "csipm_mask_index_missing_noalloc: n = %lld exceeds its maximum of %lld samples "
+                "(use csipm_mask_index_missing)",
+                (long long)n, (long long)CSIPM_MASK_NOALLOC_MAX);
+  if (n == 0) return CSIPM_OK;
+  hipStream_t st = (hipStream_t)stream;
+  CHK(no_capture(st, "csipm_mask_index_missing_noalloc"));
+  const int64_t nb = (n + per - 1) / per;   // nb + 1 <= CSIPM_MASK_NOALLOC_WORDS
+  int64_t* d = nullptr;
+  CHK(mask_scratch_ptr(&d));
+  std::lock_guard<std::mutex> lk(g_capture_mu);   // synchronises
   CHK(mask_index_launch(weights, n, index_out, d, nb, st));
   int64_t tot = 0;
   HIPCHK(hipMemcpyAsync(&tot, d + nb, sizeof(int64_t), hipMemcpyDeviceToHost, st));

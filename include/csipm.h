@@ -84,6 +84,19 @@ int csipm_op_destroy(csipm_op_t h);
  * after a copy to the host.                                               */
 int csipm_mask_index_missing(const double* weights, int64_t n, int64_t* index_out,
                              int64_t* count, void* stream);
+/* csipm_mask_index_missing without any device allocation.  No counterpart
+ * in the reference: it is for callers that cannot allocate device memory
+ * (serving loops that must not hipMalloc, harnesses that forbid it).  Same
+ * arguments, result and synchronisation as csipm_mask_index_missing; its
+ * count workspace is a fixed array of CSIPM_MASK_NOALLOC_WORDS int64 in each
+ * device's global memory (512 KiB, held until the process ends), so n is
+ * limited to CSIPM_MASK_NOALLOC_MAX samples (CSIPM_ERR_ARG beyond).  Every
+ * call on a device shares that device's array: calls that overlap on one
+ * device (from several threads or streams) race.                          */
+#define CSIPM_MASK_NOALLOC_WORDS 65536
+#define CSIPM_MASK_NOALLOC_MAX ((int64_t)(CSIPM_MASK_NOALLOC_WORDS - 1) * 4096)
+int csipm_mask_index_missing_noalloc(const double* weights, int64_t n, int64_t* index_out,
+                                     int64_t* count, void* stream);
 
 /* Device bytes handle h holds itself: plans, sampling and slot masks, the
  * scratch spectrum, partial sums, Bluestein tables, lazily made entry-point
