@@ -147,6 +147,18 @@ def host_cpu():
     return model, len(cores) or logical, logical, usable
 
 
+def cpu_threads():
+    """Threads of the CPU baseline: every CPU this process may run on
+    (sched_getaffinity), bounded by the host's CPU share for this GPU where
+    the environment states one (OMP_NUM_THREADS)."""
+    usable = host_cpu()[3]
+    try:
+        share = int(os.environ.get("OMP_NUM_THREADS", "0"))
+    except ValueError:
+        share = 0
+    return max(1, min(usable, share) if share > 0 else usable)
+
+
 def cpu_solve_time(problem, tol, workers, runs, warm=1, budget_s=None):
     """Median seconds of oracle FP64 solves (construction of b = M^T z0 and
     the solve; signal, mask and FFT plans excluded), after `warm` untimed
@@ -419,7 +431,7 @@ def run_stream(args):
     }
     if not args.no_cpu_baseline:
         frac = {1: 0.70, 2: 0.25, 3: 0.05}
-        threads = max(1, min(16, host_cpu()[3]))
+        threads = cpu_threads()
         out["cpu_baseline"] = cpu_baseline_stream([probs[k] for k in idx], frac, threads)
     print(json.dumps(out), flush=True)
     return 0
@@ -568,7 +580,7 @@ def run_batch_workload(args):
     bs.close()
     if not args.no_cpu_baseline:
         applies = res_p[0].cg_iterations + 3 * res_p[0].iterations
-        threads = max(1, min(16, host_cpu()[3]))
+        threads = cpu_threads()
         cb, f_ref = cpu_baseline_batch(problems[0], tol, threads, applies_per_solve=applies)
         model, phys, logical, usable = host_cpu()
         cb.update({"host_cpu": model, "host_physical_cores": phys, "cpus_usable": usable})
@@ -588,9 +600,9 @@ def main():
     ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--max-instances", type=int, default=1024)
     ap.add_argument("--prep-workers", type=int, default=4)
-    ap.add_argument("--latency-requests", type=int, default=1000)
-    ap.add_argument("--ref-rule-requests", type=int, default=200)
-    ap.add_argument("--plain-cap-s", type=float, default=240.0,
+    ap.add_argument("--latency-requests", type=int, default=500)
+    ap.add_argument("--ref-rule-requests", type=int, default=100)
+    ap.add_argument("--plain-cap-s", type=float, default=100.0,
                     help="stop the plain arm after this many seconds (reported)")
     ap.add_argument("--roofline-only", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")

@@ -223,13 +223,16 @@ def test_serve_rejects_oversize(cuda):
     slots_big, _ = footprint(big.dims, len(big.missing), True)
     slots_small = max(footprint(p.dims, len(p.missing), True)[0] for p in small)
     assert slots_big > 3 * slots_small
-    # a pool that holds two small requests but never the big one
-    budget = int(pool_bytes(2 * slots_small, True) / 0.75) + (64 << 20)
+    # a pool that holds two small requests but never the big one, beside a
+    # handle share of 256 MB (the pool gets pool_fraction of the budget)
+    pool = pool_bytes(2 * slots_small, True)
+    assert pool < pool_bytes(slots_big, True)
+    budget = pool + (256 << 20)
     probs = small[:2] + [big] + small[2:]
     reqs = [Request(0.001 * k, k, p) for k, p in enumerate(probs)]
     t0 = __import__("time").perf_counter()
-    out = serve(reqs, lambda p: _op(p, True), mixed=True, budget_bytes=budget, tol=1e-7,
-                prep_workers=2)
+    out = serve(reqs, lambda p: _op(p, True), mixed=True, budget_bytes=budget,
+                pool_fraction=pool / budget, tol=1e-7, prep_workers=2)
     assert __import__("time").perf_counter() - t0 < 60
     got = {r.tag.seq: r for r in out}
     assert sorted(got) == list(range(len(probs)))

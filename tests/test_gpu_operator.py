@@ -64,15 +64,21 @@ def test_operator_golden(cuda, fname):
     assert _rel(M_perpt_z(o, z).cpu().numpy(), d["m_perpt_z"]) <= TOL64
 
 
-# sizes with a prime factor above 17 run through Bluestein line transforms
-BLUESTEIN = [(127,), (4001,), (19, 23), (127, 60), (23, 19, 29), (12, 10, 37)]
+# lengths whose prime factors are at most 31 factor directly (radices 19..31)
+LARGE_RADIX = [(19, 23), (23, 19, 29), (31, 62), (57, 87)]
+# a prime length whose L - 1 factors runs Rader's form (including dimension 1
+# of a 2D/3D problem and the long-line kernel of 1D N = 4001)
+RADER = [(127,), (4001,), (191, 127), (127, 60), (12, 10, 37), (37, 41, 47), (53, 59, 61)]
+# other lengths with a prime factor above 31 run Bluestein's chirp-z form with
+# a factorable convolution length M >= 2L - 1 (4007: M = 8064 on the 1024-thread kernel)
+BLUESTEIN = [(83,), (4007,), (111, 20), (20, 111), (9, 8, 83), (107, 13, 10)]
 # 1D lines above 4096 run as four-step transforms: direct when N factors into
 # the supported radices, else with an outer Bluestein convolution
 FOURSTEP = [(8192,), (50000,), (65536,), (4097,), (12289,), (65521,)]
 
 
-@pytest.mark.parametrize("dims", [(1024,), (255, 255), (64, 48, 30), (51, 45, 35)] + BLUESTEIN +
-                         FOURSTEP)
+@pytest.mark.parametrize("dims", [(1024,), (255, 255), (64, 48, 30), (51, 45, 35)] + LARGE_RADIX +
+                         RADER + BLUESTEIN + FOURSTEP)
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
 def test_operator_masked_vs_oracle(cuda, dims, dtype):
     from csipm.fft_operator import M_perp_beta, M_perpt_M_perp_vec, M_perpt_z

@@ -103,7 +103,7 @@ def _support(beta):
 
 @pytest.mark.parametrize("mixed", [False, True])
 @pytest.mark.parametrize("case", ["ex1d", "ex2d", "ex3d", "s1d", "b1d", "b2d", "b3d", "p1d",
-                                  "p2d", "big1d", "bigp1d"])
+                                  "p2d", "bs2d", "r3d", "big1d", "bigp1d"])
 def test_solve_vs_oracle(cuda, case, mixed):
     import csipm.workloads as wl
     from csipm.model import ipm_solve
@@ -117,6 +117,8 @@ def test_solve_vs_oracle(cuda, case, mixed):
          "b3d": lambda: wl.batch_instance((26, 21, 15), seed=13),
          "p1d": lambda: wl.batch_instance((2003,), seed=14),
          "p2d": lambda: wl.batch_instance((61, 53), seed=15),
+         "bs2d": lambda: wl.batch_instance((83, 111), seed=18),
+         "r3d": lambda: wl.batch_instance((37, 23, 19), seed=19),
          "big1d": lambda: wl.batch_instance((20000,), seed=16),
          "bigp1d": lambda: wl.batch_instance((12289,), seed=17)}[case]()
     b = op.m_perpt_z(p.z0, p.dims)
