@@ -135,8 +135,10 @@ struct csipm_op {
   MapGeom g{};
   int H1 = 0;
   int64_t N = 0, Nc = 0, U = 0;  // U = 2*Nc: length of the spectrum-ordered copy of beta
-  FftPlan plan[3]{};             // line FFT of each axis (length M for Bluestein axes)
-  bool bs[3] = {false, false, false};
+  FftPlan plan[3]{};             // line FFT of each axis (Bluestein: length M; Rader: L - 1)
+  int bs[3] = {0, 0, 0};         // line transform of each axis: 0 direct, 1 Bluestein, 2 Rader
+  int rows[3] = {1, 1, 1};       // LDS rows per line (L, M, or 2L + 1 for Rader)
+  int* rperm[3] = {nullptr, nullptr, nullptr};   // Rader permutation tables (2L ints)
   // 1D lines longer than one workgroup holds: four-step over B rows x A
   // columns of the length-F transform (F = N, or Bluestein's M when obs)
   int rdft = 1;                  // as created (0: the complex path's full-spectrum maps)
@@ -178,8 +180,10 @@ static uint32_t magic(int d) {
   return d <= 1 ? 0u : (uint32_t)((((uint64_t)1 << 32) + (uint64_t)d - 1) / (uint64_t)d);
 }
 
-// radix sequence for length L: 16, 8, 4, 2 for the power of two, then odd
-// radices 17, 13, 11, 7, 15 (=3*5), 9, 5, 3
+// radix sequence for length L: the odd radices 31, 29, 23, 19 first (stage 0
+// has no twiddles, and these radices load theirs where they use them), then
+// 16, 8, 4, 2 for the power of two, then odd radices 17, 13, 11, 7, 15 (=3*5),
+// 9, 5, 3
 static bool factorize(int L, FftPlan& p) {
   p.L = L;
   p.nstages = 0;
@@ -189,6 +193,8 @@ static bool factorize(int L, FftPlan& p) {
     p.radix[p.nstages++] = r;
     return true;
   };
+  for (int r : {31, 29, 23, 19})
+    while (m % r == 0) { if (!push(r)) return false; m /= r; }
   while (m % 16 == 0) { if (!push(16)) return false; m /= 16; }
   while (m % 8 == 0) { if (!push(8)) return false; m /= 8; }
   while (m % 4 == 0) { if (!push(4)) return false; m /= 4; }
@@ -210,15 +216,33 @@ static bool factorize(int L, FftPlan& p) {
   return true;
 }
 
+static bool is_prime(int n) {
+  if (n < 2) return false;
+  for (int d = 2; (int64_t)d * d <= n; ++d)
+    if (n % d == 0) return false;
+  return true;
+}
+
 // axis plan: the direct mixed-radix plan when L factors into the supported
-// radices, else Bluestein with the power-of-two convolution length M >= 2L-1
-static bool plan_axis(int L, FftPlan& p, bool& bs) {
-  bs = false;
+// radices; for a prime L whose L - 1 does, Rader's form (plan of length
+// L - 1); else Bluestein with the smallest convolution length M >= 2L - 1
+// that factors.  rows: LDS rows a line needs.
+static bool plan_axis(int L, FftPlan& p, int& bs, int& rows) {
+  bs = 0;
+  rows = L;
   if (factorize(L, p)) return true;
-  int M = 1;
-  while (M < 2 * L - 1) M <<= 1;
-  bs = true;
-  return factorize(M, p);
+  if (L > 2 && is_prime(L) && factorize(L - 1, p)) {
+    bs = 2;
+    rows = 2 * L + 1;
+    return true;
+  }
+  bs = 1;
+  for (int M = 2 * L - 1; M <= 4 * L; ++M)
+    if (factorize(M, p)) {
+      rows = M;
+      return true;
+    }
+  return false;
 }
 
 // exp(-2 pi t i / L) for t < L (L a multiple of 4 or not), in extended
@@ -285,6 +309,46 @@ static void host_fft_pow2(std::vector<std::complex<double>>& a,
   }
 }
 
+// in-place forward DFT (exp(-2 pi i jk/M)) of any length M, host FP64, by
+// recursive decimation in time over its prime factors (M smooth: the lengths
+// the planner emits); roots = exp(-2 pi i t / M), t < M
+static void host_fft_rec(const std::complex<double>* in, int64_t stride, int n,
+                         std::complex<double>* out, const std::vector<std::complex<double>>& roots,
+                         int64_t rstep) {
+  if (n == 1) {
+    out[0] = in[0];
+    return;
+  }
+  int r = 2;
+  while (n % r) ++r;
+  const int m = n / r;
+  std::vector<std::complex<double>> sub((size_t)n);
+  for (int j = 0; j < r; ++j)
+    host_fft_rec(in + j * stride, stride * r, m, sub.data() + (size_t)j * m, roots, rstep * r);
+  // out[k + m q] = sum_j W_n^(j (k + m q)) sub_j[k], W_n^t = roots[t * rstep]
+  for (int k = 0; k < m; ++k)
+    for (int q = 0; q < r; ++q) {
+      std::complex<double> acc = 0.0;
+      const int64_t kk = k + (int64_t)m * q;
+      for (int j = 0; j < r; ++j)
+        acc += sub[(size_t)j * m + k] * roots[(size_t)(((int64_t)j * kk) % n * rstep)];
+      out[kk] = acc;
+    }
+}
+
+static void host_fft_any(std::vector<std::complex<double>>& a) {
+  const size_t M = a.size();
+  if ((M & (M - 1)) == 0) {
+    host_fft_pow2(a, *host_roots(M));
+    return;
+  }
+  std::vector<std::complex<double>> roots;
+  unit_roots((int64_t)M, roots, (int64_t)M);
+  std::vector<std::complex<double>> out(M);
+  host_fft_rec(a.data(), 1, (int)M, out.data(), roots, 1);
+  a.swap(out);
+}
+
 // Bluestein tables of line length L, convolution length M: the chirp
 // w_n = exp(-i pi (n^2 mod 2L) / L) and the kernel spectra FFT_M(b)/M of the
 // forward (b = conj w) and inverse (b = w) convolution kernels, wrapped so
@@ -317,13 +381,12 @@ static std::shared_ptr<const BluesteinHost> bluestein_host(int L, int M) {
   }
   // kernel spectra: b is symmetric (b_{M-m} = b_m), so FFT(conj b) =
   // conj(FFT(b)) and one transform gives both
-  const auto roots = host_roots((size_t)M);
   std::vector<std::complex<double>> b((size_t)M, 0.0);
   for (int m = 0; m < L; ++m) {
     b[(size_t)m] = t.w[(size_t)m];
     if (m > 0) b[(size_t)(M - m)] = t.w[(size_t)m];
   }
-  host_fft_pow2(b, *roots);
+  host_fft_any(b);
   for (auto& x : b) x /= (double)M;
   t.bf.resize(b.size());
   for (size_t i = 0; i < b.size(); ++i) t.bf[i] = std::conj(b[i]);
@@ -347,6 +410,97 @@ template <typename T>
 static int upload_bluestein(int L, int M, void** chirp, void** bfw, void** binv) {
   const std::shared_ptr<const BluesteinHost> t = bluestein_host(L, M);
   CHK(upload_cpx<T>(t->w, chirp));
+  CHK(upload_cpx<T>(t->bf, bfw));
+  CHK(upload_cpx<T>(t->bi, binv));
+  return CSIPM_OK;
+}
+
+// Rader tables of a prime length p with primitive root g: the permutation
+// perm[e] = q for e = g^q mod p (e = 1..p-1), perm[0] = p - 1 (the x_0 row
+// after the p - 1 convolution rows),
+// perm[p + m] = g^-m mod p (m = 0..p-2), and the kernel spectra
+// FFT_{p-1}(b) / (p-1) of b_j = exp(-+2 pi i g^-j / p) (forward / inverse),
+// in FP64; the last length's tables are kept (both precisions of a mixed
+// handle, and the axes of a cube, share one computation)
+struct RaderHost {
+  int p = 0;
+  std::vector<int> perm;
+  std::vector<std::complex<double>> bf, bi;
+};
+
+static int primitive_root(int p) {
+  std::vector<int> f;
+  int m = p - 1;
+  for (int d = 2; (int64_t)d * d <= m; ++d)
+    if (m % d == 0) {
+      f.push_back(d);
+      while (m % d == 0) m /= d;
+    }
+  if (m > 1) f.push_back(m);
+  auto pw = [p](int64_t b, int64_t e) {
+    int64_t r = 1;
+    b %= p;
+    for (; e; e >>= 1, b = b * b % p)
+      if (e & 1) r = r * b % p;
+    return r;
+  };
+  for (int g = 2; g < p; ++g) {
+    bool ok = true;
+    for (int q : f) ok = ok && pw(g, (p - 1) / q) != 1;
+    if (ok) return g;
+  }
+  return 1;   // p = 2 (never planned)
+}
+
+static std::shared_ptr<const RaderHost> rader_host(int p) {
+  static std::mutex mu;
+  static std::shared_ptr<const RaderHost> last;
+  {
+    std::lock_guard<std::mutex> lock(mu);
+    if (last && last->p == p) return last;
+  }
+  auto tp = std::make_shared<RaderHost>();
+  RaderHost& t = *tp;
+  t.p = p;
+  const int g = primitive_root(p);
+  int64_t ginv = 1;   // g^-1 = g^(p-2)
+  for (int k = 0; k < p - 2; ++k) ginv = ginv * g % p;
+  t.perm.assign((size_t)2 * p, 0);
+  t.perm[0] = p - 1;
+  int64_t e = 1, einv = 1;
+  std::vector<std::complex<double>> roots;
+  unit_roots(p, roots, p);   // exp(-2 pi i t / p)
+  std::vector<std::complex<double>> bf((size_t)p - 1), bi((size_t)p - 1);
+  for (int q = 0; q < p - 1; ++q) {
+    t.perm[(size_t)e] = q;                  // e = g^q
+    t.perm[(size_t)p + q] = (int)einv;      // g^-q
+    bf[(size_t)q] = roots[(size_t)einv];
+    bi[(size_t)q] = std::conj(roots[(size_t)einv]);
+    e = e * g % p;
+    einv = einv * ginv % p;
+  }
+  host_fft_any(bf);
+  host_fft_any(bi);
+  for (auto& x : bf) x /= (double)(p - 1);
+  for (auto& x : bi) x /= (double)(p - 1);
+  t.bf.swap(bf);
+  t.bi.swap(bi);
+  std::lock_guard<std::mutex> lock(mu);
+  last = tp;
+  return last;
+}
+
+static int upload_rader_perm(int p, int** perm) {
+  const std::shared_ptr<const RaderHost> t = rader_host(p);
+  std::lock_guard<std::mutex> lk(g_capture_mu);
+  HIPCHK(hipMalloc((void**)perm, sizeof(int) * t->perm.size()));
+  HIPCHK(hipMemcpy(*perm, t->perm.data(), sizeof(int) * t->perm.size(), hipMemcpyHostToDevice));
+  return CSIPM_OK;
+}
+
+template <typename T>
+static int upload_rader(int p, void** bfw, void** binv) {
+  const std::shared_ptr<const RaderHost> t = rader_host(p);
   CHK(upload_cpx<T>(t->bf, bfw));
   CHK(upload_cpx<T>(t->bi, binv));
   return CSIPM_OK;
@@ -461,15 +615,17 @@ static int min_tile_lg(size_t cs) {
   return lg;
 }
 
-// line FFTs along dim 2/3: 2^lg lines of length L per workgroup, as many as
-// the registers (lines*L <= NT*E) and 64 KB of LDS allow, fewer when that
+// line FFTs along dim 2/3: 2^lg lines per workgroup, as many as the
+// registers (lines * W <= NT*E for the FFT length W) and 64 KB of LDS (lines *
+// R rows) allow, fewer when that
 // would leave the launch with under min_grid workgroups (1024 for one
 // instance; a batch shares the target among its instances)
 
-static LineCfg line_cfg(int H1, int L, int64_t others, size_t cs, int min_grid = 1024) {
+static LineCfg line_cfg(int H1, int W, int R, int64_t others, size_t cs,
+                        int min_grid = 1024) {
   LineCfg c{};
   int lg = 0;
-  while ((2 << lg) * L <= kNT * kE && (size_t)(2 << lg) * L * cs <= kLdsSmall && (1 << lg) < H1)
+  while ((2 << lg) * W <= kNT * kE && (size_t)(2 << lg) * R * cs <= kLdsSmall && (1 << lg) < H1)
     ++lg;
   // below 64-byte rows only while the launch would not even give a
   // quarter of the workgroup target (small single instances: latency first)
@@ -480,7 +636,7 @@ static LineCfg line_cfg(int H1, int L, int64_t others, size_t cs, int min_grid =
   c.lg = lg;
   c.ntile1 = (H1 + (1 << lg) - 1) >> lg;
   c.grid = (int)(c.ntile1 * others);
-  c.lds = (size_t)(1 << lg) * L * cs;
+  c.lds = (size_t)(1 << lg) * R * cs;
   return c;
 }
 
@@ -515,13 +671,13 @@ static Dim1Cfg row_cfg(int A, int B, size_t cs, int min_grid = 1024) {
   return c;
 }
 
-static Dim1Cfg dim1_cfg(int N1, int nlines, size_t cs, int min_grid = 1024) {
+static Dim1Cfg dim1_cfg(int W, int R, int nlines, size_t cs, int min_grid = 1024) {
   Dim1Cfg c{};
   c.npairs = (nlines + 1) / 2;
-  c.nt = (N1 <= kNT * kE) ? kNT : kNTBig;
+  c.nt = (W <= kNT * kE) ? kNT : kNTBig;
   int lg = 0;
-  auto lds_of = [&](int l) { return (size_t)N1 * ((1 << l) + (l > 0 ? 1 : 0)) * cs; };
-  while ((2 << lg) * N1 <= c.nt * kE && lds_of(lg + 1) <= kLdsSmall && (1 << lg) < c.npairs) ++lg;
+  auto lds_of = [&](int l) { return (size_t)R * ((1 << l) + (l > 0 ? 1 : 0)) * cs; };
+  while ((2 << lg) * W <= c.nt * kE && lds_of(lg + 1) <= kLdsSmall && (1 << lg) < c.npairs) ++lg;
   while (lg > 0 && ((c.npairs + (1 << lg) - 1) >> lg) < min_grid) --lg;
   c.lg = lg;
   c.ps = (1 << lg) + (lg > 0 ? 1 : 0);
@@ -571,7 +727,8 @@ PassArgs<T> base_args(csipm_op* h, int axis) {
   a.n1m = magic(h->n[0]);
   a.miss = h->miss;
   a.len = h->n[axis];
-  a.bs = h->bs[axis] ? 1 : 0;
+  a.bs = h->bs[axis];
+  a.rperm = h->rperm[axis];
   a.chirp = (const cpx<T>*)h->ps<T>().chirp[axis];
   a.bfw = (const cpx<T>*)h->ps<T>().bfw[axis];
   a.binv = (const cpx<T>*)h->ps<T>().binv[axis];
@@ -1361,6 +1518,8 @@ static void destroy_impl(csipm_op_t h) {
     for (int i = 0; i < 3; ++i)
       for (void* b : {ps->chirp[i], ps->bfw[i], ps->binv[i]})
         if (b) (void)hipFree(b);
+  for (int i = 0; i < 3; ++i)
+    if (h->rperm[i]) (void)hipFree(h->rperm[i]);
   delete h->solo;
   void* bufs[] = {h->miss, h->valid, h->buf, h->part, h->scratch, h->gz, h->dplan};
   for (void* b : bufs)
@@ -1432,17 +1591,19 @@ static int set_shapes(csipm_op* h, PrecState& ps, size_t cs) {
       return fail(CSIPM_ERR_UNSUPPORTED, "four-step plan of length %d exceeds the LDS", h->n[0]);
     return CSIPM_OK;
   }
-  const int F0 = h->plan[0].L;   // FFT length (line length, or Bluestein's M)
-  ps.d1 = dim1_cfg(F0, (int)(h->N / h->n[0]), cs);
+  // FFT length W (line length, Bluestein's M, Rader's L - 1) and LDS rows of a
+  // dim-1 line (L, M, Rader's 2L + 1)
+  const int F0 = h->plan[0].L;
+  ps.d1 = dim1_cfg(F0, h->rows[0], (int)(h->N / h->n[0]), cs);
   if (ps.d1.nt * kE < F0 || ps.d1.lds > kLdsMax)
     return fail(CSIPM_ERR_UNSUPPORTED, "dimension-1 length %d (FFT length %d) exceeds one "
                 "workgroup", h->n[0], F0);
-  if (dim >= 2) ps.lines[1] = line_cfg(h->H1, h->plan[1].L, h->n[2], cs);
-  if (dim == 3) ps.lines[2] = line_cfg(h->H1, h->plan[2].L, h->n[1], cs);
+  if (dim >= 2) ps.lines[1] = line_cfg(h->H1, h->plan[1].L, h->rows[1], h->n[2], cs);
+  if (dim == 3) ps.lines[2] = line_cfg(h->H1, h->plan[2].L, h->rows[2], h->n[1], cs);
   for (int ax = 1; ax < dim; ++ax)
     if (h->plan[ax].L > kNT * kE || ps.lines[ax].lds > kLdsMax)
-      return fail(CSIPM_ERR_UNSUPPORTED, "dimension-%d length %d (FFT length %d) exceeds one "
-                  "workgroup", ax + 1, h->n[ax], h->plan[ax].L);
+      return fail(CSIPM_ERR_UNSUPPORTED, "dimension-%d length %d (%d LDS rows) exceeds one "
+                  "workgroup", ax + 1, h->n[ax], h->rows[ax]);
   return CSIPM_OK;
 }
 
@@ -1477,7 +1638,7 @@ static int plan_impl(csipm_op* h, int dim, const int64_t sizes[3], int rdft, int
   h->U = 2 * h->Nc;
   if (dim == 1 && h->n[0] > kBig1D) CHK(plan_big1d(h));
   for (int i = 0; i < dim && !h->big; ++i)
-    if (!plan_axis(h->n[i], h->plan[i], h->bs[i]))
+    if (!plan_axis(h->n[i], h->plan[i], h->bs[i], h->rows[i]))
       return fail(CSIPM_ERR_UNSUPPORTED, "no FFT plan for size %d", h->n[i]);
   if (h->has64()) CHK(set_shapes(h, h->p64, sizeof(cpx<double>)));
   if (h->has32()) CHK(set_shapes(h, h->p32, sizeof(cpx<float>)));
@@ -1505,8 +1666,10 @@ static size_t handle_own_bytes(const csipm_op* h) {
     if (h->big) {
       if (h->obs) b += cs * ((size_t)h->n[0] + 2 * (size_t)h->bF);
     } else {
-      for (int i = 0; i < h->dim; ++i)
-        if (h->bs[i]) b += cs * ((size_t)h->n[i] + 2 * (size_t)h->plan[i].L);
+      for (int i = 0; i < h->dim; ++i) {
+        if (h->bs[i] == 1) b += cs * ((size_t)h->n[i] + 2 * (size_t)h->plan[i].L);
+        if (h->bs[i] == 2) b += cs * 2 * (size_t)h->plan[i].L + 8 * (size_t)h->n[i];
+      }
     }
   }
   return b + h->scratch_bytes + h->gz_bytes;
@@ -1524,7 +1687,12 @@ static int create_impl(csipm_op* h, int dim, const int64_t sizes[3], int rdft, i
     const int F = h->plan[i].L;
     if (h->has64()) CHK(upload_twiddles<double>(F, &h->p64.tw[i]));
     if (h->has32()) CHK(upload_twiddles<float>(F, &h->p32.tw[i]));
-    if (h->bs[i]) {
+    if (h->bs[i] == 2) {
+      CHK(upload_rader_perm(h->n[i], &h->rperm[i]));
+      if (h->has64()) CHK(upload_rader<double>(h->n[i], &h->p64.bfw[i], &h->p64.binv[i]));
+      if (h->has32()) CHK(upload_rader<float>(h->n[i], &h->p32.bfw[i], &h->p32.binv[i]));
+    }
+    if (h->bs[i] == 1) {
       if (h->has64())
         CHK(upload_bluestein<double>(h->n[i], F, &h->p64.chirp[i], &h->p64.bfw[i],
                                      &h->p64.binv[i]));

@@ -462,17 +462,21 @@ __device__ __forceinline__ void stockham_stage(cpx<T>* __restrict__ lds, const F
   int tid = (int)threadIdx.x;
   asm volatile("" : "+v"(tid));
   cpx<T> v[NQ][R];
-  cpx<T> w[NQ][R];
-  // twiddle loads first (index clamped into the table for idle lanes): their
-  // latency overlaps the LDS reads and the barrier instead of following them
-  if (Ns > 1) {
+  // radices up to 17: twiddle loads first (index clamped into the table for
+  // idle lanes), so that their latency overlaps the LDS reads and the barrier
+  // instead of following them.  Radices 19..31 load each twiddle where it is
+  // used: holding R of them beside the R inputs would exceed the registers
+  // of the FP64 kernels.
+  constexpr bool kAhead = R <= 17;
+  cpx<T> w[NQ][kAhead ? R : 1];
+  if (kAhead && Ns > 1) {
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
       const int b = min(tid + q * NT, total - 1);
       const int j = b >> lg;
       const int step = (j - fdiv(j, nsm) * Ns) * tws;
 #pragma unroll
-      for (int r = 1; r < R; ++r) w[q][r] = tw[r * step];
+      for (int r = 1; r < R; ++r) w[q][kAhead ? r : 0] = tw[r * step];
     }
   }
 #pragma unroll
@@ -498,7 +502,7 @@ __device__ __forceinline__ void stockham_stage(cpx<T>* __restrict__ lds, const F
         k = j - fdiv(j, nsm) * Ns;
 #pragma unroll
         for (int r = 1; r < R; ++r) {
-          cpx<T> t = w[q][r];
+          cpx<T> t = kAhead ? w[q][kAhead ? r : 0] : tw[r * k * tws];
           if (!FWD) t.y = -t.y;
           v[q][r] = cmul(v[q][r], t);
         }
@@ -537,6 +541,10 @@ __device__ __forceinline__ void lds_fft(cpx<T>* lds, const FftPlan& plan, int lg
       case 15: stockham_stage<T, 15, E, NT, FWD>(lds, plan, s, lg, ps, tw); break;
       case 16: stockham_stage<T, 16, E, NT, FWD>(lds, plan, s, lg, ps, tw); break;
       case 17: stockham_stage<T, 17, E, NT, FWD>(lds, plan, s, lg, ps, tw); break;
+      case 19: stockham_stage<T, 19, E, NT, FWD>(lds, plan, s, lg, ps, tw); break;
+      case 23: stockham_stage<T, 23, E, NT, FWD>(lds, plan, s, lg, ps, tw); break;
+      case 29: stockham_stage<T, 29, E, NT, FWD>(lds, plan, s, lg, ps, tw); break;
+      case 31: stockham_stage<T, 31, E, NT, FWD>(lds, plan, s, lg, ps, tw); break;
       default: break;  // the host planner only emits the radices above
     }
   }

@@ -410,9 +410,9 @@ BatchCfg batch_cfg(csipm_op* h, int members) {
     b.d1.nt = kNT;
     return b;
   }
-  b.d1 = dim1_cfg(h->plan[0].L, (int)(h->N / h->n[0]), cs, target);
-  if (h->dim >= 2) b.lines[1] = line_cfg(h->H1, h->plan[1].L, h->n[2], cs, target);
-  if (h->dim == 3) b.lines[2] = line_cfg(h->H1, h->plan[2].L, h->n[1], cs, target);
+  b.d1 = dim1_cfg(h->plan[0].L, h->rows[0], (int)(h->N / h->n[0]), cs, target);
+  if (h->dim >= 2) b.lines[1] = line_cfg(h->H1, h->plan[1].L, h->rows[1], h->n[2], cs, target);
+  if (h->dim == 3) b.lines[2] = line_cfg(h->H1, h->plan[2].L, h->rows[2], h->n[1], cs, target);
   // a tile of at most kNTSmall*kE elements runs on kNTSmall threads: a stage
   // of radix R has L*lines/R butterflies, and on 256 threads a 255-point
   // tile of 8 lines (120 radix-17 butterflies) would leave half the lanes idle

@@ -395,12 +395,16 @@ struct PassArgs {
   T scale;
   const int* done;
   // line transform: length len; bs = 1 selects Bluestein's chirp-z form, in
-  // which case plan is the power-of-two convolution FFT of length M >= 2 len - 1
+  // which case plan is the convolution FFT of length M >= 2 len - 1; bs = 2
+  // selects Rader's form for a prime len, plan being the FFT of length len - 1
   int len;
   int bs;
   const cpx<T>* chirp;  // w_n = exp(-i pi n^2 / len), n < len
-  const cpx<T>* bfw;    // FFT_M(b) / M of the forward kernel b = conj(w) (wrapped)
-  const cpx<T>* binv;   // the same for the inverse kernel b = w
+  const cpx<T>* bfw;    // FFT_M(b) / M of the forward kernel b = conj(w) (wrapped);
+                        // Rader: FFT_{L-1}(b) / (L-1) of b_j = exp(-2 pi i g^-j / L)
+  const cpx<T>* binv;   // the same for the inverse kernel b = w (Rader: conj b)
+  const int* rperm;     // Rader: [e] row of element e in the convolution order
+                        // (e = 0: row L), [L + m] element g^-m mod L of output m
   // long 1D lines (four-step): the length-F transform (F = len, or the
   // Bluestein length when obs) as A columns x B rows, position col + A*row
   int A, B, F;
@@ -414,10 +418,66 @@ struct PassArgs {
 // factor above 17): X_k = w_k sum_n (x_n w_n) conj(w_{k-n}) (forward; the
 // inverse conjugates w), the convolution by FFT_M, pointwise product with the
 // precomputed kernel spectrum, inverse FFT_M.  LDS must hold M rows.
+// Rader (prime L = p, bs = 2), with g a primitive root mod p:
+//   X_0 = x_0 + sum_{n>0} x_n,   X_{g^-m} = x_0 + sum_q x_{g^q} w^{g^(q-m)}
+// (w = exp(-+2 pi i / p)), a cyclic convolution of length p - 1 of
+// a_q = x_{g^q} with b_j = w^{g^-j}, done by FFT_{p-1}, the pointwise product
+// with the precomputed FFT_{p-1}(b) / (p - 1) and the inverse FFT_{p-1}.
+// LDS rows of a tile: 0..p-1 the line in natural order, p..2p-2 the
+// convolution (row p + q), 2p-1 x_0 and 2p X_0 (= x_0 + A_0, the DC term of
+// FFT_{p-1}(a)).  The two permutations move between disjoint rows, one
+// element at a time.
+template <typename T, int E, int NT, bool FWD>
+__device__ __forceinline__ void rader_dft(cpx<T>* lds, const PassArgs<T>& a, int lg, int ps) {
+  const int L = a.len, Lm = L - 1, nl = 1 << lg;
+  const int* __restrict__ perm = a.rperm;
+  cpx<T>* conv = lds + L * ps;
+  cpx<T>* x0row = lds + (2 * L - 1) * ps;
+  cpx<T>* X0row = lds + 2 * L * ps;
+  // an opaque copy of the lane index per phase keeps each phase's table
+  // loads inside it (hoisted above the transforms they would hold registers)
+  int tid = (int)threadIdx.x;
+  asm volatile("" : "+v"(tid));
+  for (int idx = tid; idx < (L << lg); idx += NT) {
+    const int line = idx & (nl - 1), e = idx >> lg;
+    const int q = perm[e];   // e = 0: q = p - 1, the x_0 row
+    conv[q * ps + line] = lds[e * ps + line];
+  }
+  __syncthreads();
+  lds_fft<T, E, NT, true>(conv, *a.plan, lg, ps, a.tw);
+  const cpx<T>* __restrict__ B = FWD ? a.bfw : a.binv;
+  int tid2 = (int)threadIdx.x;
+  asm volatile("" : "+v"(tid2));
+  for (int idx = tid2; idx < (Lm << lg); idx += NT) {
+    const int line = idx & (nl - 1), e = idx >> lg;
+    cpx<T>* q = conv + e * ps + line;
+    const cpx<T> v = *q;
+    if (e == 0) X0row[line] = cadd(x0row[line], v);
+    *q = cmul(v, B[e]);
+  }
+  __syncthreads();
+  lds_fft<T, E, NT, false>(conv, *a.plan, lg, ps, a.tw);
+  int tid3 = (int)threadIdx.x;
+  asm volatile("" : "+v"(tid3));
+  for (int idx = tid3; idx < (L << lg); idx += NT) {
+    const int line = idx & (nl - 1), m = idx >> lg;
+    // X_{g^-m} = x_0 + c_m into row g^-m (1..p-1); the last index: X_0 into row 0
+    if (m < Lm)
+      lds[perm[L + m] * ps + line] = cadd(conv[m * ps + line], x0row[line]);
+    else
+      lds[line] = X0row[line];
+  }
+  __syncthreads();
+}
+
 template <typename T, int E, int NT, bool FWD>
 __device__ __forceinline__ void line_dft(cpx<T>* lds, const PassArgs<T>& a, int lg, int ps) {
   if (!a.bs) {
     lds_fft<T, E, NT, FWD>(lds, *a.plan, lg, ps, a.tw);
+    return;
+  }
+  if (a.bs == 2) {
+    rader_dft<T, E, NT, FWD>(lds, a, lg, ps);
     return;
   }
   const int L = a.len, M = uniform(a.plan->L), nl = 1 << lg;

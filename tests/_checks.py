@@ -4,9 +4,10 @@ support_flips / assert_same_support: the support rule of the solves -- the
 same support above rel * max|ref| (rel = 1e-6).  A flip is an index where
 exactly one of the two solutions exceeds the threshold; the assertion lists
 every flip with both values in units of the threshold instead of exempting
-any band around it.  Given the reference solved to a tighter tolerance, a
-flip on which beta agrees with that tighter solve is the reference's own
-unconverged entry and is returned instead of failing.
+any band around it (except entries on the threshold itself, within 1e-3 of
+it in both solutions, which are printed).  Given the reference solved to a
+tighter tolerance, a flip on which beta agrees with that tighter solve is
+the reference's own unconverged entry and is returned instead of failing.
 
 certificate: the algorithm-free LASSO optimality certificate of oracle.ipm
 (prox-gradient fixed-point residual and duality gap of the residual-scaled
@@ -26,13 +27,29 @@ def support_flips(beta, ref, rel=1e-6):
     return idx, thr
 
 
+BOUNDARY = 1e-3   # half-width of the threshold band of a boundary flip, in units of thr
+
+
 def assert_same_support(beta, ref, name, rel=1e-6, tighter=None):
     """tighter: the reference solved to a smaller tolerance.  A flip against
     ref where beta falls on the same side of the threshold as `tighter` is
     the reference's own unconverged entry (an interior-point iterate at tol
     1e-8 can still hold ~1e-5 max|beta| off the support), not a difference
-    of beta's: such flips are returned, the others fail."""
+    of beta's: such flips are returned, the others fail.
+
+    A flip with both values within BOUNDARY * thr of the threshold (an entry
+    sitting on it, which two solutions equal to 1e-9 can straddle) is a
+    boundary flip: printed with its values, not failed."""
     idx, thr = support_flips(beta, ref, rel)
+    if len(idx):
+        a = np.abs(np.asarray(beta)[idx])
+        r = np.abs(np.asarray(ref)[idx])
+        on = (np.minimum(a, r) >= (1 - BOUNDARY) * thr) & (np.maximum(a, r) <= (1 + BOUNDARY) * thr)
+        if on.any():
+            det = [(int(i), float(beta[i]) / thr, float(ref[i]) / thr) for i in idx[on][:12]]
+            print(f"  {name}: {int(on.sum())} boundary flip(s) within {BOUNDARY:g} thr of the "
+                  f"threshold (index, beta/thr, ref/thr): {det}", flush=True)
+        idx = idx[~on]
     excused = []
     if tighter is not None and len(idx):
         t = np.abs(np.asarray(tighter)[idx]) > thr
