@@ -141,10 +141,16 @@ class BatchSolver:
         _lib.call("csipm_batch_poll", self._b, ids, cap, ctypes.byref(n))
         return [ids[k] for k in range(min(n.value, cap))]
 
-    def harvest(self, i) -> Result:
+    def harvest(self, i, retire=None) -> Result:
+        """Results of finished instance i (its beta is written in stream
+        order).  retire: a list that receives the instance's operator handle
+        instead of letting it go here (a server destroys retired handles off
+        its critical path)."""
         info = (ctypes.c_double * 8)()
         _lib.call("csipm_batch_harvest", self._b, int(i), info)
         op, z, beta, tag, t0 = self._live.pop(i)
+        if retire is not None:
+            retire.append(op)
         return Result(beta, info[0], int(info[1]), int(info[2]), _STATUS[int(info[3])], info[4],
                       info[5], tag, t0, time.perf_counter())
 
@@ -205,7 +211,8 @@ class Request:
 
 def serve(requests, make_op, mixed=True, budget_bytes=200 << 30, pool_fraction=0.75,
           max_instances=512, tol=1e-6, max_iter=200, cg_rtol=1e-10, cg_atol=0.0,
-          clock=time.perf_counter, on_done=None, prep_workers=4, scan=64):
+          clock=time.perf_counter, on_done=None, prep_workers=4, scan=64, stats=None,
+          solver=None):
     """Continuous batching over an arrival stream.
 
     requests: iterable of Request (arrival time in seconds from the start,
@@ -224,12 +231,24 @@ def serve(requests, make_op, mixed=True, budget_bytes=200 << 30, pool_fraction=0
     Every loop turn runs one interior-point iteration of all admitted
     problems and retires the finished ones (their handles are freed at
     once).  Returns the Results (tag = the request), admit / finish times
-    relative to the start."""
+    relative to the start.  stats: a dict that receives the engine's
+    accounting (BatchSolver.stats) and the loop's own split of the wall time
+    (seconds in steps, admission, harvest, idle waits).
+
+    solver: an empty BatchSolver of the same precision to serve with (a
+    server's pool, made once at start-up and kept across calls; its slots
+    replace pool_fraction * budget_bytes, the handles get the rest of the
+    budget); by default serve makes one and frees it on return."""
     from collections import deque
     from concurrent.futures import FIRST_COMPLETED, ThreadPoolExecutor
     from concurrent.futures import wait as fwait
     per_slot = pool_bytes(1, mixed)
-    pool_slots = max(1, int(budget_bytes * pool_fraction) // per_slot)
+    if solver is not None:
+        if solver.mixed != bool(mixed) or solver.live:
+            raise ValueError("serve needs an empty solver of the requested precision")
+        pool_slots = solver.slots
+    else:
+        pool_slots = max(1, int(budget_bytes * pool_fraction) // per_slot)
     handle_budget = budget_bytes - pool_slots * per_slot
     pending = list(requests)
     heapq.heapify(pending)
@@ -249,8 +268,11 @@ def serve(requests, make_op, mixed=True, budget_bytes=200 << 30, pool_fraction=0
             on_done(r)
 
     prep = ThreadPoolExecutor(max_workers=prep_workers) if prep_workers > 0 else None
-    bs = BatchSolver(pool_slots, max_instances=max_instances, mixed=mixed)
+    bs = solver if solver is not None else \
+        BatchSolver(pool_slots, max_instances=max_instances, mixed=mixed)
     t_start = clock()
+    acc = {"step_s": 0.0, "admit_s": 0.0, "harvest_s": 0.0, "idle_s": 0.0, "steps": 0,
+           "live_sum": 0}
     try:
         while pending or waiting or bs.live:
             now = clock() - t_start
@@ -277,6 +299,7 @@ def serve(requests, make_op, mixed=True, budget_bytes=200 << 30, pool_fraction=0
                 w[1] = prep.submit(prepare, w[0].problem) if prep is not None else \
                     _made(prepare, w[0].problem)
             # admit ready requests in arrival order
+            t_a = time.perf_counter()
             skipped = 0
             admitted = 0
             k = 0
@@ -316,7 +339,9 @@ def serve(requests, make_op, mixed=True, budget_bytes=200 << 30, pool_fraction=0
                 live_bytes[req.seq] = w[2]
                 del waiting[k]
                 admitted += 1
+            acc["admit_s"] += time.perf_counter() - t_a
             if bs.live == 0:
+                t_i = time.perf_counter()
                 futs = [w[1] for w in waiting if w[1] is not None and not w[1].done()]
                 nxt = pending[0].arrival - (clock() - t_start) if pending else None
                 if futs:
@@ -324,16 +349,39 @@ def serve(requests, make_op, mixed=True, budget_bytes=200 << 30, pool_fraction=0
                           return_when=FIRST_COMPLETED)
                 elif nxt is not None and nxt > 0:
                     time.sleep(nxt)
+                acc["idle_s"] += time.perf_counter() - t_i
                 continue
+            t_s = time.perf_counter()
+            acc["live_sum"] += bs.live
             bs.step(tol, max_iter, cg_rtol, cg_atol)
+            t_h = time.perf_counter()
+            acc["step_s"] += t_h - t_s
+            acc["steps"] += 1
+            retired = []
             for i in bs.finished():
-                r = bs.harvest(i)
+                r = bs.harvest(i, retire=retired)
                 r.t_admit -= t_start
                 r.t_done = clock() - t_start
                 held -= live_bytes.pop(r.tag.seq, 0)
                 finish(r)
+            # the retired handles are destroyed on a helper thread, beside the
+            # next step (destruction synchronises the handle's stream)
+            if retired:
+                if prep is not None:
+                    prep.submit(_close_all, retired)
+                else:
+                    _close_all(retired)
+            acc["harvest_s"] += time.perf_counter() - t_h
     finally:
-        bs.close()
+        if stats is not None:
+            try:
+                stats.update(bs.stats())
+            except Exception:
+                pass
+            stats.update({"loop_" + k: v for k, v in acc.items()})
+            stats["wall_s"] = clock() - t_start
+        if solver is None:
+            bs.close()
         if prep is not None:
             prep.shutdown(wait=True, cancel_futures=True)
     return out
@@ -354,6 +402,11 @@ class _Ready:
         if self.exc is not None:
             raise self.exc
         return self.op
+
+
+def _close_all(ops):
+    for op in ops:
+        op.close()
 
 
 def _made(fn, *args):

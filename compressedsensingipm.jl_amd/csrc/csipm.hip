@@ -85,6 +85,150 @@ static std::mutex g_capture_mu;
   } while (0)
 
 // ---------------------------------------------------------------------------
+// Device blocks of operator handles and engines (plans, masks, scratch
+// spectra, FFT tables, engine vectors) come from a per-device cache of freed
+// blocks in power-of-two size classes: a serving loop that makes and destroys
+// a handle per request reuses blocks instead of calling hipMalloc / hipFree
+// (which synchronises the whole device) for each.  Blocks above
+// kCacheMaxBlock bypass the cache, at most kCacheKeepBytes per device stay
+// cached, and an allocation that fails empties the cache and retries.  The
+// owners make sure no work that uses a block is in flight when they return
+// it (handles and engines synchronise their stream first).
+// ---------------------------------------------------------------------------
+constexpr size_t kCacheMaxBlock = size_t(1) << 30;
+constexpr size_t kCacheKeepBytes = size_t(4) << 30;
+
+struct BlockCache {
+  std::mutex mu;
+  std::unordered_map<int64_t, std::vector<void*>> free;   // (device, class) -> blocks
+  std::unordered_map<void*, int64_t> owner;               // cached-size block -> key
+  std::unordered_map<int, size_t> cached;                 // device -> cached bytes
+};
+static BlockCache g_blocks;
+
+static int block_class(size_t bytes) {
+  int c = 8;
+  while (((size_t)1 << c) < bytes) ++c;
+  return c;
+}
+
+static hipError_t dmalloc(void** p, size_t bytes) {
+  *p = nullptr;
+  if (bytes == 0) bytes = 1;
+  if (bytes > kCacheMaxBlock) return hipMalloc(p, bytes);
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  const int c = block_class(bytes);
+  const int64_t key = ((int64_t)dev << 8) | c;
+  {
+    std::lock_guard<std::mutex> lk(g_blocks.mu);
+    auto it = g_blocks.free.find(key);
+    if (it != g_blocks.free.end() && !it->second.empty()) {
+      *p = it->second.back();
+      it->second.pop_back();
+      g_blocks.cached[dev] -= (size_t)1 << c;
+      return hipSuccess;
+    }
+  }
+  e = hipMalloc(p, (size_t)1 << c);
+  if (e == hipErrorOutOfMemory) {
+    (void)hipGetLastError();
+    std::vector<void*> drop;
+    {
+      std::lock_guard<std::mutex> lk(g_blocks.mu);
+      for (auto& kv : g_blocks.free)
+        if ((kv.first >> 8) == dev) {
+          for (void* b : kv.second) {
+            drop.push_back(b);
+            g_blocks.owner.erase(b);
+          }
+          kv.second.clear();
+        }
+      g_blocks.cached[dev] = 0;
+    }
+    for (void* b : drop) (void)hipFree(b);
+    e = hipMalloc(p, (size_t)1 << c);
+  }
+  if (e == hipSuccess) {
+    std::lock_guard<std::mutex> lk(g_blocks.mu);
+    g_blocks.owner[*p] = key;
+  }
+  return e;
+}
+
+// pinned host blocks of engines (flag mirrors, table staging): a smaller
+// cache of the same kind, so that the batch-of-one engine of every
+// csipm_solve does not pin and unpin host memory
+constexpr size_t kPinnedKeepBytes = size_t(256) << 20;
+struct PinnedCache {
+  std::mutex mu;
+  std::unordered_map<int, std::vector<void*>> free;   // class -> blocks
+  std::unordered_map<void*, int> owner;
+  size_t cached = 0;
+};
+static PinnedCache g_pinned;
+
+static hipError_t hmalloc(void** p, size_t bytes) {
+  *p = nullptr;
+  const int c = block_class(bytes == 0 ? 1 : bytes);
+  {
+    std::lock_guard<std::mutex> lk(g_pinned.mu);
+    auto it = g_pinned.free.find(c);
+    if (it != g_pinned.free.end() && !it->second.empty()) {
+      *p = it->second.back();
+      it->second.pop_back();
+      g_pinned.cached -= (size_t)1 << c;
+      return hipSuccess;
+    }
+  }
+  const hipError_t e = hipHostMalloc(p, (size_t)1 << c, hipHostMallocDefault);
+  if (e == hipSuccess) {
+    std::lock_guard<std::mutex> lk(g_pinned.mu);
+    g_pinned.owner[*p] = c;
+  }
+  return e;
+}
+
+static void hfree(void* p) {
+  if (!p) return;
+  {
+    std::lock_guard<std::mutex> lk(g_pinned.mu);
+    auto it = g_pinned.owner.find(p);
+    if (it != g_pinned.owner.end()) {
+      const size_t sz = (size_t)1 << it->second;
+      if (g_pinned.cached + sz <= kPinnedKeepBytes) {
+        g_pinned.free[it->second].push_back(p);
+        g_pinned.cached += sz;
+        return;
+      }
+      g_pinned.owner.erase(it);
+    }
+  }
+  (void)hipHostFree(p);
+}
+
+static void dfree(void* p) {
+  if (!p) return;
+  {
+    std::lock_guard<std::mutex> lk(g_blocks.mu);
+    auto it = g_blocks.owner.find(p);
+    if (it != g_blocks.owner.end()) {
+      const int64_t key = it->second;
+      const int dev = (int)(key >> 8);
+      const size_t sz = (size_t)1 << (key & 255);
+      if (g_blocks.cached[dev] + sz <= kCacheKeepBytes) {
+        g_blocks.free[key].push_back(p);
+        g_blocks.cached[dev] += sz;
+        return;
+      }
+      g_blocks.owner.erase(it);
+    }
+  }
+  (void)hipFree(p);
+}
+
+// ---------------------------------------------------------------------------
 // handle
 // ---------------------------------------------------------------------------
 namespace {
@@ -401,7 +545,7 @@ static int upload_cpx(const std::vector<std::complex<double>>& v, void** out) {
   std::vector<cpx<T>> h(v.size());
   for (size_t i = 0; i < v.size(); ++i) h[i] = {(T)v[i].real(), (T)v[i].imag()};
   std::lock_guard<std::mutex> lk(g_capture_mu);
-  HIPCHK(hipMalloc(out, sizeof(cpx<T>) * h.size()));
+  HIPCHK(dmalloc(out, sizeof(cpx<T>) * h.size()));
   HIPCHK(hipMemcpy(*out, h.data(), sizeof(cpx<T>) * h.size(), hipMemcpyHostToDevice));
   return CSIPM_OK;
 }
@@ -493,7 +637,7 @@ static std::shared_ptr<const RaderHost> rader_host(int p) {
 static int upload_rader_perm(int p, int** perm) {
   const std::shared_ptr<const RaderHost> t = rader_host(p);
   std::lock_guard<std::mutex> lk(g_capture_mu);
-  HIPCHK(hipMalloc((void**)perm, sizeof(int) * t->perm.size()));
+  HIPCHK(dmalloc((void**)perm, sizeof(int) * t->perm.size()));
   HIPCHK(hipMemcpy(*perm, t->perm.data(), sizeof(int) * t->perm.size(), hipMemcpyHostToDevice));
   return CSIPM_OK;
 }
@@ -1265,11 +1409,11 @@ static int gz_workspace(csipm_op* h, int nvec, T** out) {
     std::lock_guard<std::mutex> lk(g_capture_mu);
     if (h->gz) {
       HIPCHK(hipStreamSynchronize(h->stream));
-      HIPCHK(hipFree(h->gz));
+      dfree(h->gz);
       h->gz = nullptr;
       h->gz_bytes = 0;
     }
-    HIPCHK(hipMalloc(&h->gz, need));
+    HIPCHK(dmalloc(&h->gz, need));
     h->gz_bytes = need;
   }
   *out = (T*)h->gz;
@@ -1513,17 +1657,17 @@ static void destroy_impl(csipm_op_t h) {
     release_twiddles(ps->twA);
     release_twiddles(ps->twB);
   }
+  // work of this handle still in flight on its stream finishes before its
+  // blocks go back to the cache (engines synchronise their own stream)
+  (void)hipStreamSynchronize(h->stream);
   std::lock_guard<std::mutex> lk(g_capture_mu);
+  delete h->solo;
   for (PrecState* ps : {&h->p64, &h->p32})
     for (int i = 0; i < 3; ++i)
-      for (void* b : {ps->chirp[i], ps->bfw[i], ps->binv[i]})
-        if (b) (void)hipFree(b);
-  for (int i = 0; i < 3; ++i)
-    if (h->rperm[i]) (void)hipFree(h->rperm[i]);
-  delete h->solo;
+      for (void* b : {ps->chirp[i], ps->bfw[i], ps->binv[i]}) dfree(b);
+  for (int i = 0; i < 3; ++i) dfree(h->rperm[i]);
   void* bufs[] = {h->miss, h->valid, h->buf, h->part, h->scratch, h->gz, h->dplan};
-  for (void* b : bufs)
-    if (b) (void)hipFree(b);
+  for (void* b : bufs) dfree(b);
   delete h;
 }
 
@@ -1703,7 +1847,7 @@ static int create_impl(csipm_op* h, int dim, const int64_t sizes[3], int rdft, i
   }
   {
     const FftPlan plans[5] = {h->plan[0], h->plan[1], h->plan[2], h->planA, h->planB};
-    HIPL(hipMalloc(&h->dplan, sizeof(plans)));
+    HIPL(dmalloc((void**)&h->dplan, sizeof(plans)));
     HIPL(hipMemcpy(h->dplan, plans, sizeof(plans), hipMemcpyHostToDevice));
   }
   // missing-sample bitmask
@@ -1718,13 +1862,13 @@ static int create_impl(csipm_op* h, int dim, const int64_t sizes[3], int rdft, i
                     (long long)h->N);
       bits[(size_t)(s >> 5)] |= 1u << (s & 31);
     }
-    HIPL(hipMalloc(&h->miss, sizeof(uint32_t) * (size_t)words));
+    HIPL(dmalloc((void**)&h->miss, sizeof(uint32_t) * (size_t)words));
     HIPL(hipMemcpy(h->miss, bits.data(), sizeof(uint32_t) * (size_t)words,
                      hipMemcpyHostToDevice));
   }
   // u-slot validity
   const int64_t vwords = (h->U + 31) / 32;
-  HIPL(hipMalloc(&h->valid, sizeof(uint32_t) * (size_t)vwords));
+  HIPL(dmalloc((void**)&h->valid, sizeof(uint32_t) * (size_t)vwords));
   {
     std::lock_guard<std::mutex> lk(g_capture_mu);
     const int grid = (int)((vwords + kNT - 1) / kNT);
@@ -1736,8 +1880,8 @@ static int create_impl(csipm_op* h, int dim, const int64_t sizes[3], int rdft, i
       hipLaunchKernelGGL((k_u_valid<3>), dim3(grid), dim3(kNT), 0, h->stream, h->valid, h->g, h->H1, h->Nc);
     HIPCHK(hipGetLastError());
   }
-  HIPL(hipMalloc(&h->buf, (size_t)std::max<int64_t>(h->Nc, h->bF) * sizeof(cpx<double>)));
-  HIPL(hipMalloc(&h->part, sizeof(double) * (size_t)h->npart));
+  HIPL(dmalloc(&h->buf, (size_t)std::max<int64_t>(h->Nc, h->bF) * sizeof(cpx<double>)));
+  HIPL(dmalloc((void**)&h->part, sizeof(double) * (size_t)h->npart));
   HIPL(hipStreamSynchronize(h->stream));
   return CSIPM_OK;
 }
@@ -2144,7 +2288,7 @@ extern "C++" {
 static int nlp_scratch(csipm_op* h) {
   if (h->scratch) return CSIPM_OK;
   CHK(no_capture(h->stream, "an NLP-model entry point"));
-  HIPL(hipMalloc(&h->scratch, h->esize() * (size_t)h->N));
+  HIPL(dmalloc(&h->scratch, h->esize() * (size_t)h->N));
   h->scratch_bytes = h->esize() * (size_t)h->N;
   return CSIPM_OK;
 }
@@ -2416,7 +2560,7 @@ int csipm_batch_harvest(csipm_batch_t b, int id, double* info) {
   if (id < 0 || id >= E->maxinst || !E->inst[(size_t)id].h || E->inst[(size_t)id].fresh)
     return fail(CSIPM_ERR_ARG, "no admitted instance %d", id);
   CHK(engine_enter(E));
-  CHK(engine_harvest(E, id, info));
+  CHK(engine_harvest(E, id, info, false));
   return engine_leave(E);
 }
 
@@ -2442,7 +2586,9 @@ int csipm_batch_solve(csipm_batch_t b, int n, const csipm_op_t* ops, const doubl
     }
     if (running == 0) break;
   }
-  for (int i = 0; i < n; ++i) CHK(engine_harvest(b->E, ids[(size_t)i], info + 8 * (size_t)i));
+  for (int i = 0; i < n; ++i)
+    CHK(engine_harvest(b->E, ids[(size_t)i], info + 8 * (size_t)i, false));
+  HIPCHK(hipStreamSynchronize(b->E->stream));
   return engine_leave(b->E);
 }
 

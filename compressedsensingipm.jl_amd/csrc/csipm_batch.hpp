@@ -34,6 +34,13 @@ struct BFlags {
   double f, dual, mu;
 };
 
+// zeroed interior-point flags of newly admitted instances (ids[0, n))
+__global__ __launch_bounds__(256) void bk_reset_flags(const int* __restrict__ ids, int n,
+                                                      BFlags* flags) {
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i < n) flags[ids[i]] = BFlags{};
+}
+
 #define BSEG_BEGIN                                                  \
   const int inst = uniform(seg.inst[blockIdx.x]);                   \
   const int f0_ = uniform(seg.first[inst]);                         \

@@ -143,6 +143,62 @@ int launch_spec(const PassSpec& s, int grid, size_t lds, hipStream_t st, const v
               s.src, s.dst, s.fm);
 }
 
+// Launch tables of a rebuild (chain pass arguments, block -> instance maps,
+// vector segments) are staged on the host, packed into one pinned buffer and
+// copied to one device buffer by a single asynchronous copy on the engine's
+// stream.  Both buffers are kept and reused by the next rebuild: the copy is
+// ordered after every earlier launch that reads the old tables, and the host
+// buffer is reused once the event after the previous copy has passed, so a
+// rebuild allocates, frees and synchronises nothing unless the tables
+// outgrow the buffers.
+struct TableArena {
+  char* dev = nullptr;
+  char* host = nullptr;
+  size_t cap = 0;
+  hipEvent_t ev = nullptr;      // recorded after the last copy out of `host`
+  std::vector<char> stage;      // the tables of the rebuild in progress
+  void begin() { stage.clear(); }
+  // appends bytes (256-byte aligned); returns their offset
+  size_t put(const void* p, size_t bytes) {
+    const size_t off = (stage.size() + 255) & ~(size_t)255;
+    stage.resize(off + bytes);
+    if (bytes) std::memcpy(stage.data() + off, p, bytes);
+    return off;
+  }
+  // copies the staged tables to the device; *base: their device address
+  int commit(hipStream_t st, char** base) {
+    const size_t need = std::max<size_t>(stage.size(), 256);
+    if (ev) HIPCHK(hipEventSynchronize(ev));
+    if (need > cap) {
+      // launches still reading the old tables finish before the buffer goes
+      HIPCHK(hipStreamSynchronize(st));
+      dfree(dev);
+      hfree(host);
+      dev = nullptr;
+      host = nullptr;
+      cap = 0;
+      const size_t want = (size_t)1 << block_class(std::max(need, (size_t)1 << 14) * 2);
+      HIPCHK(dmalloc((void**)&dev, want));
+      HIPCHK(hmalloc((void**)&host, want));
+      cap = want;
+    }
+    if (!ev) HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    std::memcpy(host, stage.data(), stage.size());
+    HIPCHK(hipMemcpyAsync(dev, host, stage.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipEventRecord(ev, st));
+    *base = dev;
+    return CSIPM_OK;
+  }
+  ~TableArena() {
+    if (ev) {
+      (void)hipEventSynchronize(ev);
+      (void)hipEventDestroy(ev);
+    }
+    dfree(dev);
+    hfree(host);
+  }
+};
+
 // one launch of a chain: a pass over a subset of instances
 struct ChainSub {
   PassSpec spec;
@@ -150,20 +206,35 @@ struct ChainSub {
   std::vector<int> ids;  // member instance ids
   int nt = kNT, grid = 0;
   size_t lds = 0;
-  void* tab = nullptr;   // device PassArgs<T>[members]
+  void* tab = nullptr;   // device PassArgs<T>[members] (in a table arena)
   int* seg = nullptr;    // device: block -> member, then member -> first block
+  size_t tab_off = 0, seg_off = 0;   // their offsets in the arena
   int members = 0;
 };
 
 struct Chain {
   int esize = 8;
   std::vector<ChainSub> subs;
-  void release() {
+  void release() { subs.clear(); }   // the tables live in a table arena
+  // device addresses of the tables once the arena is committed
+  void bind(char* base) {
     for (ChainSub& s : subs) {
-      if (s.tab) (void)hipFree(s.tab);
-      if (s.seg) (void)hipFree(s.seg);
+      s.tab = base + s.tab_off;
+      s.seg = (int*)(base + s.seg_off);
     }
-    subs.clear();
+  }
+};
+
+// a vector segment staged in a table arena: block -> instance, instance ->
+// first block (maxinst + 1 entries), then the id list
+struct VsegOff {
+  size_t off = 0;
+  int nblk = 0, nfirst = 0, grid = 0;
+  void bind(char* base, BSeg* seg, const int** ids) const {
+    int* m = (int*)(base + off);
+    seg->inst = m;
+    seg->first = m + nblk;
+    *ids = m + nblk + nfirst;
   }
 };
 
@@ -247,7 +318,12 @@ struct Engine {
   void* Wc = nullptr;        // C_NV1 * cap + C_NV2 * 2 cap CG-precision elements
   // launch tables of the current active set
   std::vector<int> active;   // instance ids
-  int* seg_mem = nullptr;    // vector segment (block -> inst, inst -> first block) + ids
+  TableArena tabs;           // the active set's tables
+  TableArena init_tabs;      // the tables of the fresh-instance initialisation
+  TableArena rec_tabs;       // staged instance records (info_h) and new ids
+  std::vector<int> pending;  // admitted since the last record upload (flags to zero)
+  bool records_dirty = false;
+  bool have_tables = false;
   int vgrid = 0;
   BSeg vseg{};
   const int* ids = nullptr;
@@ -263,8 +339,7 @@ struct Engine {
 
   void release_tables() {
     for (Chain& c : chains) c.release();
-    if (seg_mem) (void)hipFree(seg_mem);
-    seg_mem = nullptr;
+    have_tables = false;
   }
   void release_graphs() {
     for (auto& g : graph)
@@ -279,13 +354,15 @@ struct Engine {
       if (in.h) in.h->admitted.fetch_sub(1);
     release_graphs();
     release_tables();
+    // blocks go back to the device-block cache: the stream's work ends first
+    if (stream) (void)hipStreamSynchronize(stream);
     if (stream) (void)hipStreamDestroy(stream);
     if (ev) (void)hipEventDestroy(ev);
     for (void* p : {(void*)info, (void*)flags, (void*)cg, (void*)scal, (void*)part, (void*)valid,
                     (void*)Wd, Wc})
-      if (p) (void)hipFree(p);
-    if (flags_h) (void)hipHostFree(flags_h);
-    if (cg_h) (void)hipHostFree(cg_h);
+      dfree(p);
+    hfree(flags_h);
+    hfree(cg_h);
   }
 };
 
@@ -328,16 +405,16 @@ int engine_create(Engine** out, int64_t cap_slots, int maxinst, int tc_size, boo
   } while (0)
   ECHK(hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking));
   ECHK(hipEventCreateWithFlags(&E->ev, hipEventDisableTiming));
-  ECHK(hipMalloc(&E->info, sizeof(BInst) * (size_t)maxinst));
-  ECHK(hipMalloc(&E->flags, sizeof(BFlags) * (size_t)maxinst));
-  ECHK(hipHostMalloc(&E->flags_h, sizeof(BFlags) * (size_t)maxinst, hipHostMallocDefault));
-  ECHK(hipMalloc(&E->cg, sizeof(CgState) * (size_t)maxinst));
-  ECHK(hipHostMalloc(&E->cg_h, sizeof(CgState) * (size_t)maxinst, hipHostMallocDefault));
-  ECHK(hipMalloc(&E->scal, sizeof(double) * S_NSLOTS * (size_t)maxinst));
-  ECHK(hipMalloc(&E->part, sizeof(double) * (size_t)(E->cap / kSlotAlign * kPartPerAlign)));
-  ECHK(hipMalloc(&E->valid, sizeof(uint32_t) * (size_t)(E->cap / 32)));
-  ECHK(hipMalloc(&E->Wd, sizeof(double) * (size_t)E->cap * (V_NV1 + 2 * V_NV2)));
-  ECHK(hipMalloc(&E->Wc, (size_t)tc_size * (size_t)E->cap * (C_NV1 + 2 * C_NV2)));
+  ECHK(dmalloc((void**)&E->info, sizeof(BInst) * (size_t)maxinst));
+  ECHK(dmalloc((void**)&E->flags, sizeof(BFlags) * (size_t)maxinst));
+  ECHK(hmalloc((void**)&E->flags_h, sizeof(BFlags) * (size_t)maxinst));
+  ECHK(dmalloc((void**)&E->cg, sizeof(CgState) * (size_t)maxinst));
+  ECHK(hmalloc((void**)&E->cg_h, sizeof(CgState) * (size_t)maxinst));
+  ECHK(dmalloc((void**)&E->scal, sizeof(double) * S_NSLOTS * (size_t)maxinst));
+  ECHK(dmalloc((void**)&E->part, sizeof(double) * (size_t)(E->cap / kSlotAlign * kPartPerAlign)));
+  ECHK(dmalloc((void**)&E->valid, sizeof(uint32_t) * (size_t)(E->cap / 32)));
+  ECHK(dmalloc((void**)&E->Wd, sizeof(double) * (size_t)E->cap * (V_NV1 + 2 * V_NV2)));
+  ECHK(dmalloc(&E->Wc, (size_t)tc_size * (size_t)E->cap * (C_NV1 + 2 * C_NV2)));
 #undef ECHK
   *out = E;
   return CSIPM_OK;
@@ -492,7 +569,8 @@ inline int pass_nt(const PassSpec& s, const BatchCfg& bc) {
 }
 
 template <typename T>
-int build_chain(Engine* E, const std::vector<int>& ids, int call, int nb, Chain& out) {
+int build_chain(Engine* E, TableArena& A, const std::vector<int>& ids, int call, int nb,
+                Chain& out) {
   out.release();
   out.esize = sizeof(T);
   for (int key : {1, 2, 3, 10, 11}) {
@@ -554,18 +632,12 @@ int build_chain(Engine* E, const std::vector<int>& ids, int call, int nb, Chain&
         first.push_back(tot);
         sub.grid = tot;
         segv.insert(segv.end(), first.begin(), first.end());
-        HIPCHK(hipMalloc(&sub.tab, sizeof(PassArgs<T>) * tab.size()));
-        HIPCHK(hipMalloc(&sub.seg, sizeof(int) * segv.size()));
-        HIPCHK(hipMemcpyAsync(sub.tab, tab.data(), sizeof(PassArgs<T>) * tab.size(),
-                              hipMemcpyHostToDevice, E->stream));
-        HIPCHK(hipMemcpyAsync(sub.seg, segv.data(), sizeof(int) * segv.size(),
-                              hipMemcpyHostToDevice, E->stream));
+        sub.tab_off = A.put(tab.data(), sizeof(PassArgs<T>) * tab.size());
+        sub.seg_off = A.put(segv.data(), sizeof(int) * segv.size());
         out.subs.push_back(sub);
       }
     }
   }
-  // the host vectors above are freed on return: finish the copies first
-  HIPCHK(hipStreamSynchronize(E->stream));
   return CSIPM_OK;
 }
 
@@ -579,6 +651,30 @@ int cg_final_grid(csipm_op* h, int members) {
   return bc.lines[h->dim - 1].grid;
 }
 
+// the instance records (info_h) to the device and zeroed flags for the
+// instances admitted since the last upload, staged through a table arena:
+// asynchronous on the engine's stream, no synchronisation
+int engine_upload_records(Engine* E) {
+  if (!E->records_dirty && E->pending.empty()) return CSIPM_OK;
+  TableArena& A = E->rec_tabs;
+  A.begin();
+  const size_t ro = A.put(E->info_h.data(), sizeof(BInst) * E->info_h.size());
+  const size_t io = A.put(E->pending.data(), sizeof(int) * E->pending.size());
+  char* base = nullptr;
+  CHK(A.commit(E->stream, &base));
+  HIPCHK(hipMemcpyAsync(E->info, base + ro, sizeof(BInst) * E->info_h.size(),
+                        hipMemcpyDeviceToDevice, E->stream));
+  const int n = (int)E->pending.size();
+  if (n > 0) {
+    hipLaunchKernelGGL(bk_reset_flags, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, E->stream,
+                       (const int*)(base + io), n, E->flags);
+    HIPCHK(hipGetLastError());
+  }
+  E->pending.clear();
+  E->records_dirty = false;
+  return CSIPM_OK;
+}
+
 // BInst.gcg of the active instances for a CG chain over all of them
 template <typename TC>
 int update_gcg(Engine* E) {
@@ -590,15 +686,12 @@ int update_gcg(Engine* E) {
       return fail(CSIPM_ERR_UNSUPPORTED, "partial sums of instance exceed its share");
     E->info_h[(size_t)id].gcg = g;
   }
-  HIPCHK(hipMemcpyAsync(E->info, E->info_h.data(), sizeof(BInst) * E->info_h.size(),
-                        hipMemcpyHostToDevice, E->stream));
-  HIPCHK(hipStreamSynchronize(E->stream));
-  return CSIPM_OK;
+  E->records_dirty = true;
+  return engine_upload_records(E);
 }
 
-// vector segment + id list of `ids` into one device allocation
-int upload_vseg(Engine* E, const std::vector<int>& ids, int** mem, BSeg* seg, const int** idp,
-                int* grid) {
+// vector segment + id list of `ids`, staged in A
+void stage_vseg(Engine* E, TableArena& A, const std::vector<int>& ids, VsegOff* out) {
   std::vector<int> blk, first(E->maxinst + 1, 0);
   int tot = 0;
   // first[] is indexed by instance id: ids not in the set get empty ranges
@@ -613,23 +706,18 @@ int upload_vseg(Engine* E, const std::vector<int>& ids, int** mem, BSeg* seg, co
   std::vector<int> all(blk);
   all.insert(all.end(), first.begin(), first.end());
   all.insert(all.end(), ids.begin(), ids.end());
-  if (*mem) (void)hipFree(*mem);
-  *mem = nullptr;
-  HIPCHK(hipMalloc(mem, sizeof(int) * std::max<size_t>(all.size(), 1)));
-  HIPCHK(hipMemcpyAsync(*mem, all.data(), sizeof(int) * all.size(), hipMemcpyHostToDevice,
-                        E->stream));
-  HIPCHK(hipStreamSynchronize(E->stream));
-  seg->inst = *mem;
-  seg->first = *mem + blk.size();
-  *idp = *mem + blk.size() + first.size();
-  *grid = tot;
-  return CSIPM_OK;
+  all.push_back(0);   // never empty
+  out->off = A.put(all.data(), sizeof(int) * all.size());
+  out->nblk = (int)blk.size();
+  out->nfirst = (int)first.size();
+  out->grid = tot;
 }
 
 // launch tables of the active set: the vector segment and the chains of the
 // interior-point iteration (ipm = false: only the CG apply, for a plain CG)
 template <typename TC>
 int engine_tables(Engine* E, int nb, bool ipm) {
+  CHK(engine_upload_records(E));
   std::vector<int64_t> sig;
   for (int id : E->active) {
     // the admission number, not the handle address: a handle created after
@@ -640,7 +728,7 @@ int engine_tables(Engine* E, int nb, bool ipm) {
     sig.push_back(E->inst[(size_t)id].off);
   }
   if (E->tables_dirty && sig == E->table_sig && nb == E->table_nb && ipm == E->table_ipm &&
-      E->seg_mem) {
+      E->have_tables) {
     // the same instances in the same slots: the tables stand (the instance
     // records were rewritten by the admissions)
     CHK(update_gcg<TC>(E));
@@ -651,14 +739,23 @@ int engine_tables(Engine* E, int nb, bool ipm) {
   E->stats[Engine::ST_NTABLES] += 1;
   E->release_tables();
   E->table_sig.clear();
-  CHK(upload_vseg(E, E->active, &E->seg_mem, &E->vseg, &E->ids, &E->vgrid));
-  CHK(build_chain<TC>(E, E->active, CALL_CGA, nb, E->chains[CALL_CGA]));
-  CHK(build_chain<TC>(E, E->active, CALL_CGR, nb, E->chains[CALL_CGR]));
-  CHK(update_gcg<TC>(E));
+  TableArena& A = E->tabs;
+  A.begin();
+  VsegOff vs;
+  stage_vseg(E, A, E->active, &vs);
+  CHK(build_chain<TC>(E, A, E->active, CALL_CGA, nb, E->chains[CALL_CGA]));
+  CHK(build_chain<TC>(E, A, E->active, CALL_CGR, nb, E->chains[CALL_CGR]));
   if (ipm) {
-    CHK(build_chain<double>(E, E->active, CALL_QB, nb, E->chains[CALL_QB]));
-    if (E->mixed) CHK(build_chain<double>(E, E->active, CALL_REF, nb, E->chains[CALL_REF]));
+    CHK(build_chain<double>(E, A, E->active, CALL_QB, nb, E->chains[CALL_QB]));
+    if (E->mixed) CHK(build_chain<double>(E, A, E->active, CALL_REF, nb, E->chains[CALL_REF]));
   }
+  char* base = nullptr;
+  CHK(A.commit(E->stream, &base));
+  vs.bind(base, &E->vseg, &E->ids);
+  E->vgrid = vs.grid;
+  for (Chain& c : E->chains) c.bind(base);
+  E->have_tables = true;
+  CHK(update_gcg<TC>(E));
   E->tables_dirty = false;
   E->table_sig = sig;
   E->table_nb = nb;
@@ -712,14 +809,14 @@ int engine_admit(Engine* E, csipm_op* h, const double* z0, double lam, double* b
   bi.pbase = off / kSlotAlign * kPartPerAlign;
   bi.gcg = gcg;
   bi.itmax = (int)std::min<int64_t>(cg_itmax > 0 ? cg_itmax : 4 * h->N, 1 << 30);
-  BFlags fl{};
+  // the record and zeroed flags reach the device with the next record
+  // upload (one staged copy for all admissions since); the slot mask is a
+  // device-to-device copy in stream order
   E->info_h[(size_t)k] = bi;
-  HIPCHK(hipMemcpyAsync(E->info + k, &E->info_h[(size_t)k], sizeof(BInst), hipMemcpyHostToDevice,
-                        E->stream));
-  HIPCHK(hipMemcpyAsync(E->flags + k, &fl, sizeof(BFlags), hipMemcpyHostToDevice, E->stream));
+  E->pending.push_back(k);
+  E->records_dirty = true;
   HIPCHK(hipMemcpyAsync(E->valid + off / 32, h->valid, sizeof(uint32_t) * (size_t)((h->U + 31) / 32),
                         hipMemcpyDeviceToDevice, E->stream));
-  HIPCHK(hipStreamSynchronize(E->stream));
   E->active.push_back(k);
   E->tables_dirty = true;
   h->admitted.fetch_add(1);
@@ -749,20 +846,28 @@ int finalize(Engine* E, const BSeg& vseg, const int* ids, int nids, std::initial
 
 // initialise the fresh instances: b = M^T z0 (u order), ||b||_inf, the starting point
 int engine_init_fresh(Engine* E) {
+  CHK(engine_upload_records(E));
   std::vector<int> fresh;
   for (int id : E->active)
     if (E->inst[(size_t)id].fresh) fresh.push_back(id);
   if (fresh.empty()) return CSIPM_OK;
   StatTimer tm(&E->stats[Engine::ST_INIT]);
   Chain mtz;
-  int* mem = nullptr;
   BSeg seg{};
   const int* ids = nullptr;
   int grid = 0;
   int rc = CSIPM_OK;
   do {
-    if ((rc = build_chain<double>(E, fresh, CALL_MTZ, 2, mtz)) != CSIPM_OK) break;
-    if ((rc = upload_vseg(E, fresh, &mem, &seg, &ids, &grid)) != CSIPM_OK) break;
+    TableArena& A = E->init_tabs;
+    A.begin();
+    if ((rc = build_chain<double>(E, A, fresh, CALL_MTZ, 2, mtz)) != CSIPM_OK) break;
+    VsegOff vs;
+    stage_vseg(E, A, fresh, &vs);
+    char* base = nullptr;
+    if ((rc = A.commit(E->stream, &base)) != CSIPM_OK) break;
+    mtz.bind(base);
+    vs.bind(base, &seg, &ids);
+    grid = vs.grid;
     if ((rc = run_chain<double>(mtz, E->stream)) != CSIPM_OK) break;
     hipLaunchKernelGGL(bk_absmax, dim3(grid), dim3(kNT), 0, E->stream, seg, E->info, E->flags,
                        E->W(V_B), E->part);
@@ -774,13 +879,8 @@ int engine_init_fresh(Engine* E) {
       rc = fail(CSIPM_ERR_HIP, "init launch failed");
       break;
     }
-    if (hipStreamSynchronize(E->stream) != hipSuccess) {
-      rc = fail(CSIPM_ERR_HIP, "init failed");
-      break;
-    }
   } while (false);
   mtz.release();
-  if (mem) (void)hipFree(mem);
   if (rc) return rc;
   for (int id : fresh) E->inst[(size_t)id].fresh = false;
   return CSIPM_OK;
@@ -1027,13 +1127,18 @@ int engine_step(Engine* E, double tol, int max_iter, double cg_atol, double cg_r
 // results of a finished instance: beta (reference layout) into its beta_out,
 // info[8] = [objective, IPM iterations, CG iterations, status, dual residual,
 // mu, 0, 0]; the instance is retired and its slots freed
-int engine_harvest(Engine* E, int id, double* info) {
+// sync = false: the flags the last step (or poll) read are used when they
+// show the instance done (its flags no longer change), and the conversion of
+// beta is left in flight on the engine's stream
+int engine_harvest(Engine* E, int id, double* info, bool sync = true) {
   Engine::Inst& in = E->inst[(size_t)id];
   CHK(from_u<double>(in.h, E->W(V_BETA) + in.off, in.beta_out, E->stream));
   if (in.c_out) CHK(from_u<double>(in.h, E->W(V_C) + in.off, in.c_out, E->stream, 0));
-  HIPCHK(hipMemcpyAsync(E->flags_h + id, E->flags + id, sizeof(BFlags), hipMemcpyDeviceToHost,
-                        E->stream));
-  HIPCHK(hipStreamSynchronize(E->stream));
+  if (sync || !E->flags_h[id].done) {
+    HIPCHK(hipMemcpyAsync(E->flags_h + id, E->flags + id, sizeof(BFlags), hipMemcpyDeviceToHost,
+                          E->stream));
+    HIPCHK(hipStreamSynchronize(E->stream));
+  }
   const BFlags& f = E->flags_h[id];
   if (info) {
     info[0] = f.f;
@@ -1127,19 +1232,27 @@ int engine_apply_timed(Engine* E, int64_t nrep, double* ms, double* bytes, std::
 // ---------------------------------------------------------------------------
 template <typename TC>
 int engine_admm_init_fresh(Engine* E, double rho) {
+  CHK(engine_upload_records(E));
   std::vector<int> fresh;
   for (int id : E->active)
     if (E->inst[(size_t)id].fresh) fresh.push_back(id);
   if (fresh.empty()) return CSIPM_OK;
   Chain mtz;
-  int* mem = nullptr;
   BSeg seg{};
   const int* ids = nullptr;
   int grid = 0;
   int rc = CSIPM_OK;
   do {
-    if ((rc = build_chain<double>(E, fresh, CALL_MTZ, 1, mtz))) break;
-    if ((rc = upload_vseg(E, fresh, &mem, &seg, &ids, &grid))) break;
+    TableArena& A = E->init_tabs;
+    A.begin();
+    if ((rc = build_chain<double>(E, A, fresh, CALL_MTZ, 1, mtz))) break;
+    VsegOff vs;
+    stage_vseg(E, A, fresh, &vs);
+    char* base = nullptr;
+    if ((rc = A.commit(E->stream, &base))) break;
+    mtz.bind(base);
+    vs.bind(base, &seg, &ids);
+    grid = vs.grid;
     if ((rc = run_chain<double>(mtz, E->stream))) break;
     for (int id : fresh) {
       Engine::Inst& in = E->inst[(size_t)id];
@@ -1156,7 +1269,6 @@ int engine_admm_init_fresh(Engine* E, double rho) {
       rc = fail(CSIPM_ERR_HIP, "ADMM initialisation failed");
   } while (false);
   mtz.release();
-  if (mem) (void)hipFree(mem);
   if (rc) return rc;
   for (int id : fresh) E->inst[(size_t)id].fresh = false;
   return CSIPM_OK;

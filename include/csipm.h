@@ -340,7 +340,9 @@ int csipm_batch_step(csipm_batch_t b, double tol, int64_t max_iter,
                      double cg_atol, double cg_rtol, int* running);
 int csipm_batch_poll(csipm_batch_t b, int* ids, int cap, int* n);
 /* Write a finished instance's beta into its beta_out, fill info[8] as
- * csipm_solve does (seconds = 0) and free its slots and id.              */
+ * csipm_solve does (seconds = 0) and free its slots and id.  beta_out is
+ * written asynchronously, in order on the batch's stream (the host does not
+ * wait); info comes from the flags the last step read.                   */
 int csipm_batch_harvest(csipm_batch_t b, int id, double* info);
 /* Convenience: admit n problems, iterate until all finish, harvest all.
  * info: n * 8 doubles.                                                   */
