@@ -29,6 +29,10 @@ SIGNATURES = {
     "csipm_op_destroy": (_i, [_vp]),
     "csipm_op_set_stream": (_i, [_vp, _vp]),
     "csipm_op_info": (_i, [_vp, ctypes.POINTER(_i64), ctypes.POINTER(_i64), ctypes.POINTER(_i)]),
+    "csipm_op_fft_shape": (_i, [_vp, _i, ctypes.POINTER(_i), ctypes.POINTER(_i),
+                                ctypes.POINTER(_i)]),
+    "csipm_tune_set": (_i, [_i, _i, _i, _i, _i, _i]),
+    "csipm_tune_clear": (_i, []),
     "csipm_op_bytes": (_i, [_vp, ctypes.POINTER(_i64)]),
     "csipm_op_footprint": (_i, [_i, ctypes.POINTER(_i64), _i, _i64, ctypes.POINTER(_i64),
                                 ctypes.POINTER(_i64)]),
@@ -104,11 +108,17 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"libcsipm.so not found at {LIB_PATH}: run "
                                "`python -c 'import __graft_entry__ as g; g.build()'` first")
+        # torch first: libcsipm then binds to the HIP runtime torch has loaded
+        # (two runtimes in one process do not share devices)
+        import torch  # noqa: F401
         L = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             f = getattr(L, name)
             f.restype = res
             f.argtypes = args
+        # the autotuned launch shapes of the batched FFT passes (committed table)
+        from . import tune
+        tune.load(L)
         _lib = L
     return _lib
 

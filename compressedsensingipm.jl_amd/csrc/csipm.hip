@@ -10,10 +10,12 @@
 #include <cstring>
 #include <initializer_list>
 #include <limits>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <unordered_map>
 #include <string>
+#include <tuple>
 #include <complex>
 #include <vector>
 
@@ -750,6 +752,26 @@ static void release_twiddles(void* p) {
   }
 }
 
+// Launch-shape choices of the batched passes measured by the autotuner
+// (csipm.tune, persisted in a committed table and loaded at start-up): for a
+// pass kind (0: line pass, 1: dimension-1 pass), the complex element size,
+// the FFT length W and the LDS rows R of a line, the largest tile (log2 of
+// the lines or line pairs per workgroup) and the workgroup size (128 or
+// 256).  Shapes without an entry keep the built-in rules.
+struct TuneVal {
+  int lg_cap = 30;
+  int nt = 0;   // 0: the built-in choice
+};
+static std::mutex g_tune_mu;
+static std::map<std::tuple<int, int, int, int>, TuneVal> g_tune;
+
+static TuneVal tune_lookup(int kind, size_t cs, int W, int R) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  if (g_tune.empty()) return TuneVal{};
+  auto it = g_tune.find(std::make_tuple(kind, (int)cs, W, R));
+  return it == g_tune.end() ? TuneVal{} : it->second;
+}
+
 // smallest tile (lines per workgroup) whose rows span 64 bytes: with the
 // XCD-aware block order of the pass kernels, the two tiles sharing a 128-byte
 // line run on one XCD and fetch it once
@@ -766,10 +788,11 @@ static int min_tile_lg(size_t cs) {
 // instance; a batch shares the target among its instances)
 
 static LineCfg line_cfg(int H1, int W, int R, int64_t others, size_t cs,
-                        int min_grid = 1024) {
+                        int min_grid = 1024, int lg_cap = 30) {
   LineCfg c{};
   int lg = 0;
-  while ((2 << lg) * W <= kNT * kE && (size_t)(2 << lg) * R * cs <= kLdsSmall && (1 << lg) < H1)
+  while ((2 << lg) * W <= kNT * kE && (size_t)(2 << lg) * R * cs <= kLdsSmall && (1 << lg) < H1 &&
+         lg < lg_cap)
     ++lg;
   // below 64-byte rows only while the launch would not even give a
   // quarter of the workgroup target (small single instances: latency first)
@@ -815,13 +838,16 @@ static Dim1Cfg row_cfg(int A, int B, size_t cs, int min_grid = 1024) {
   return c;
 }
 
-static Dim1Cfg dim1_cfg(int W, int R, int nlines, size_t cs, int min_grid = 1024) {
+static Dim1Cfg dim1_cfg(int W, int R, int nlines, size_t cs, int min_grid = 1024,
+                        int lg_cap = 30) {
   Dim1Cfg c{};
   c.npairs = (nlines + 1) / 2;
   c.nt = (W <= kNT * kE) ? kNT : kNTBig;
   int lg = 0;
   auto lds_of = [&](int l) { return (size_t)R * ((1 << l) + (l > 0 ? 1 : 0)) * cs; };
-  while ((2 << lg) * W <= c.nt * kE && lds_of(lg + 1) <= kLdsSmall && (1 << lg) < c.npairs) ++lg;
+  while ((2 << lg) * W <= c.nt * kE && lds_of(lg + 1) <= kLdsSmall && (1 << lg) < c.npairs &&
+         lg < lg_cap)
+    ++lg;
   while (lg > 0 && ((c.npairs + (1 << lg) - 1) >> lg) < min_grid) --lg;
   c.lg = lg;
   c.ps = (1 << lg) + (lg > 0 ? 1 : 0);
@@ -1925,6 +1951,44 @@ int csipm_op_footprint(int dim, const int64_t sizes[3], int precision, int64_t n
   tmp.nmiss = n_missing;
   *slots = slots_of(&tmp);
   *bytes = (int64_t)handle_own_bytes(&tmp);
+  return CSIPM_OK;
+}
+
+int csipm_op_fft_shape(csipm_op_t h, int axis, int* fft_len, int* rows, int* mode) {
+  if (!h || !fft_len || !rows || !mode || axis < 0 || axis >= h->dim)
+    return fail(CSIPM_ERR_ARG, "bad argument");
+  if (h->big) {
+    *fft_len = h->bF;
+    *rows = h->bF;
+    *mode = 3;
+    return CSIPM_OK;
+  }
+  *fft_len = h->plan[axis].L;
+  *rows = h->rows[axis];
+  *mode = h->bs[axis];
+  return CSIPM_OK;
+}
+
+int csipm_tune_set(int kind, int elem_bytes, int fft_len, int rows, int lg_cap, int nt) {
+  if ((kind != 0 && kind != 1) || (elem_bytes != 8 && elem_bytes != 16) || fft_len < 1 ||
+      rows < fft_len || (nt != 0 && nt != kNTSmall && nt != kNT))
+    return fail(CSIPM_ERR_ARG, "bad tuning entry");
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  const auto key = std::make_tuple(kind, elem_bytes, fft_len, rows);
+  if (lg_cap < 0) {
+    g_tune.erase(key);
+    return CSIPM_OK;
+  }
+  TuneVal v;
+  v.lg_cap = lg_cap;
+  v.nt = nt;
+  g_tune[key] = v;
+  return CSIPM_OK;
+}
+
+int csipm_tune_clear(void) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tune.clear();
   return CSIPM_OK;
 }
 

@@ -487,16 +487,28 @@ BatchCfg batch_cfg(csipm_op* h, int members) {
     b.d1.nt = kNT;
     return b;
   }
-  b.d1 = dim1_cfg(h->plan[0].L, h->rows[0], (int)(h->N / h->n[0]), cs, target);
-  if (h->dim >= 2) b.lines[1] = line_cfg(h->H1, h->plan[1].L, h->rows[1], h->n[2], cs, target);
-  if (h->dim == 3) b.lines[2] = line_cfg(h->H1, h->plan[2].L, h->rows[2], h->n[1], cs, target);
+  const TuneVal t0 = tune_lookup(1, cs, h->plan[0].L, h->rows[0]);
+  b.d1 = dim1_cfg(h->plan[0].L, h->rows[0], (int)(h->N / h->n[0]), cs, target, t0.lg_cap);
+  TuneVal ta[3];
+  for (int a = 1; a < h->dim; ++a) {
+    ta[a] = tune_lookup(0, cs, h->plan[a].L, h->rows[a]);
+    b.lines[a] = line_cfg(h->H1, h->plan[a].L, h->rows[a], h->n[a == 1 ? 2 : 1], cs, target,
+                          ta[a].lg_cap);
+  }
   // a tile of at most kNTSmall*kE elements runs on kNTSmall threads: a stage
   // of radix R has L*lines/R butterflies, and on 256 threads a 255-point
-  // tile of 8 lines (120 radix-17 butterflies) would leave half the lanes idle
+  // tile of 8 lines (120 radix-17 butterflies) would leave half the lanes
+  // idle (a tuned choice replaces the rule where the tile fits it)
   auto small = [](int64_t elems) { return elems <= (int64_t)kNTSmall * kE; };
-  if (b.d1.nt == kNT && small((int64_t)h->plan[0].L << b.d1.lg)) b.d1.nt = kNTSmall;
+  auto pick = [&](int nt_rule, int64_t elems, const TuneVal& t) {
+    if (nt_rule == kNTBig) return nt_rule;
+    if (t.nt == kNTSmall && small(elems)) return kNTSmall;
+    if (t.nt == kNT) return kNT;
+    return small(elems) ? kNTSmall : kNT;
+  };
+  b.d1.nt = pick(b.d1.nt, (int64_t)h->plan[0].L << b.d1.lg, t0);
   for (int a = 1; a < h->dim; ++a)
-    b.lines[a].nt = small((int64_t)h->plan[a].L << b.lines[a].lg) ? kNTSmall : kNT;
+    b.lines[a].nt = pick(kNT, (int64_t)h->plan[a].L << b.lines[a].lg, ta[a]);
   return b;
 }
 

@@ -434,39 +434,38 @@ __device__ __forceinline__ void rader_dft(cpx<T>* lds, const PassArgs<T>& a, int
   cpx<T>* conv = lds + L * ps;
   cpx<T>* x0row = lds + (2 * L - 1) * ps;
   cpx<T>* X0row = lds + 2 * L * ps;
-  // an opaque copy of the lane index per phase keeps each phase's table
-  // loads inside it (hoisted above the transforms they would hold registers)
-  int tid = (int)threadIdx.x;
-  asm volatile("" : "+v"(tid));
-  for (int idx = tid; idx < (L << lg); idx += NT) {
-    const int line = idx & (nl - 1), e = idx >> lg;
-    const int q = perm[e];   // e = 0: q = p - 1, the x_0 row
-    conv[q * ps + line] = lds[e * ps + line];
-  }
+  // the table loads of each phase are issued in groups (staged) so that a
+  // thread has several in flight instead of waiting for each before its move
+  staged<NT, 8, int>(
+      L << lg, [&](int idx) { return perm[idx >> lg]; },   // e = 0: q = p - 1, the x_0 row
+      [&](int idx, int q) {
+        const int line = idx & (nl - 1), e = idx >> lg;
+        conv[q * ps + line] = lds[e * ps + line];
+      });
   __syncthreads();
   lds_fft<T, E, NT, true>(conv, *a.plan, lg, ps, a.tw);
   const cpx<T>* __restrict__ B = FWD ? a.bfw : a.binv;
-  int tid2 = (int)threadIdx.x;
-  asm volatile("" : "+v"(tid2));
-  for (int idx = tid2; idx < (Lm << lg); idx += NT) {
-    const int line = idx & (nl - 1), e = idx >> lg;
-    cpx<T>* q = conv + e * ps + line;
-    const cpx<T> v = *q;
-    if (e == 0) X0row[line] = cadd(x0row[line], v);
-    *q = cmul(v, B[e]);
-  }
+  staged<NT, stage_k<T>(), cpx<T>>(
+      Lm << lg, [&](int idx) { return B[idx >> lg]; },
+      [&](int idx, const cpx<T>& bk) {
+        const int line = idx & (nl - 1), e = idx >> lg;
+        cpx<T>* q = conv + e * ps + line;
+        const cpx<T> v = *q;
+        if (e == 0) X0row[line] = cadd(x0row[line], v);
+        *q = cmul(v, bk);
+      });
   __syncthreads();
   lds_fft<T, E, NT, false>(conv, *a.plan, lg, ps, a.tw);
-  int tid3 = (int)threadIdx.x;
-  asm volatile("" : "+v"(tid3));
-  for (int idx = tid3; idx < (L << lg); idx += NT) {
-    const int line = idx & (nl - 1), m = idx >> lg;
-    // X_{g^-m} = x_0 + c_m into row g^-m (1..p-1); the last index: X_0 into row 0
-    if (m < Lm)
-      lds[perm[L + m] * ps + line] = cadd(conv[m * ps + line], x0row[line]);
-    else
-      lds[line] = X0row[line];
-  }
+  // X_{g^-m} = x_0 + c_m into row g^-m (1..p-1); the last index: X_0 into row 0
+  staged<NT, 8, int>(
+      L << lg, [&](int idx) { return perm[L + min(idx >> lg, Lm - 1)]; },
+      [&](int idx, int row) {
+        const int line = idx & (nl - 1), m = idx >> lg;
+        if (m < Lm)
+          lds[row * ps + line] = cadd(conv[m * ps + line], x0row[line]);
+        else
+          lds[line] = X0row[line];
+      });
   __syncthreads();
 }
 
@@ -481,34 +480,43 @@ __device__ __forceinline__ void line_dft(cpx<T>* lds, const PassArgs<T>& a, int 
     return;
   }
   const int L = a.len, M = uniform(a.plan->L), nl = 1 << lg;
-  for (int idx = threadIdx.x; idx < (M << lg); idx += NT) {
-    const int line = idx & (nl - 1), e = idx >> lg;
-    cpx<T>* q = lds + e * ps + line;
-    if (e < L) {
-      cpx<T> w = a.chirp[e];
-      if (!FWD) w.y = -w.y;
-      *q = cmul(*q, w);
-    } else {
-      *q = {(T)0, (T)0};
-    }
-  }
+  constexpr int K = stage_k<T>();
+  // chirp and kernel-spectrum loads issued in groups (staged), as in the
+  // tile loads
+  staged<NT, K, cpx<T>>(
+      M << lg, [&](int idx) { return a.chirp[min(idx >> lg, L - 1)]; },
+      [&](int idx, const cpx<T>& wc) {
+        const int line = idx & (nl - 1), e = idx >> lg;
+        cpx<T>* q = lds + e * ps + line;
+        if (e < L) {
+          cpx<T> w = wc;
+          if (!FWD) w.y = -w.y;
+          *q = cmul(*q, w);
+        } else {
+          *q = {(T)0, (T)0};
+        }
+      });
   __syncthreads();
   lds_fft<T, E, NT, true>(lds, *a.plan, lg, ps, a.tw);
   const cpx<T>* B = FWD ? a.bfw : a.binv;
-  for (int idx = threadIdx.x; idx < (M << lg); idx += NT) {
-    const int line = idx & (nl - 1), e = idx >> lg;
-    cpx<T>* q = lds + e * ps + line;
-    *q = cmul(*q, B[e]);
-  }
+  staged<NT, K, cpx<T>>(
+      M << lg, [&](int idx) { return B[idx >> lg]; },
+      [&](int idx, const cpx<T>& bk) {
+        const int line = idx & (nl - 1), e = idx >> lg;
+        cpx<T>* q = lds + e * ps + line;
+        *q = cmul(*q, bk);
+      });
   __syncthreads();
   lds_fft<T, E, NT, false>(lds, *a.plan, lg, ps, a.tw);
-  for (int idx = threadIdx.x; idx < (L << lg); idx += NT) {
-    const int line = idx & (nl - 1), e = idx >> lg;
-    cpx<T>* q = lds + e * ps + line;
-    cpx<T> w = a.chirp[e];
-    if (!FWD) w.y = -w.y;
-    *q = cmul(*q, w);
-  }
+  staged<NT, K, cpx<T>>(
+      L << lg, [&](int idx) { return a.chirp[idx >> lg]; },
+      [&](int idx, const cpx<T>& wc) {
+        const int line = idx & (nl - 1), e = idx >> lg;
+        cpx<T>* q = lds + e * ps + line;
+        cpx<T> w = wc;
+        if (!FWD) w.y = -w.y;
+        *q = cmul(*q, w);
+      });
   __syncthreads();
 }
 

@@ -117,6 +117,24 @@ int csipm_op_footprint(int dim, const int64_t sizes[3], int precision, int64_t n
 int csipm_op_set_stream(csipm_op_t h, void* stream);
 /* Query N (length of beta) and Nc (complex entries of the half spectrum). */
 int csipm_op_info(csipm_op_t h, int64_t* n, int64_t* nc, int* precision);
+/* The line transform of axis `axis` (0-based) of h: FFT length (the line
+ * length; Bluestein's convolution length; L - 1 for Rader's form; the
+ * four-step length of long 1D lines), LDS rows per line, and the form
+ * (0 direct, 1 Bluestein, 2 Rader, 3 four-step).  No counterpart in the
+ * reference (rocFFT/FFTW plans are opaque there); the autotuner keys its
+ * table on (FFT length, rows).                                            */
+int csipm_op_fft_shape(csipm_op_t h, int axis, int* fft_len, int* rows, int* mode);
+
+/* Autotuned launch shapes of the batch engine's FFT passes (no counterpart
+ * in the reference): for pass kind 0 (line pass) or 1 (dimension-1 pass),
+ * complex element size 8 (FP32) or 16 (FP64), FFT length and LDS rows, cap
+ * the tile at 2^lg_cap lines (line pairs) per workgroup and run it on nt
+ * threads (128 or 256; 0 keeps the built-in choice); lg_cap < 0 removes the
+ * entry.  csipm.tune measures the candidates and writes the committed
+ * table the package loads at start-up.  Takes effect at the next table
+ * rebuild of an engine (a new batch, or the next change of its instances). */
+int csipm_tune_set(int kind, int elem_bytes, int fft_len, int rows, int lg_cap, int nt);
+int csipm_tune_clear(void);
 
 /* beta -> DFT map.  beta_to_DFT! (src/fft_utils.jl:195-204, per-element form
  * src/mapping_cpu_kernels.jl:353-738).  v: (M1+1)*N2*N3 complex when full==0,
