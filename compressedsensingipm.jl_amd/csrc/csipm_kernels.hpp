@@ -803,6 +803,33 @@ __device__ __forceinline__ void dim1_body(const PassArgs<T>& a, int tile) {
     __syncthreads();
     line_dft<T, E, NT, false>(lds, a, lg, ps);
     const int totr = N1 << lg;
+    if (MODE == MODE_MTM && (stage_mask || !a.miss) && np <= NT) {
+      // the mask in place from LDS: a thread keeps one line pair p (NT is a
+      // multiple of np), so its lines, their staged mask rows and the bit
+      // offset of sample 0 in their first word are fixed; per sample only
+      // the word and bit of o + n are formed
+      const int p = (int)threadIdx.x & (np - 1);
+      const int dn = NT >> lg;
+      const int la = 2 * (p0 + p), lb = la + 1;
+      const T sa0 = la < a.nlines ? a.scale : (T)0;
+      const T sb0 = lb < a.nlines ? a.scale : (T)0;
+      const int oa = (int)(((int64_t)la * N1) & 31), ob = (int)(((int64_t)lb * N1) & 31);
+      const uint32_t* ra = mw + (2 * p) * W;
+      const uint32_t* rb = ra + W;
+      cpx<T>* q = lds + ((int)threadIdx.x >> lg) * ps + p;
+      for (int n = (int)threadIdx.x >> lg; n < N1; n += dn, q += dn * ps) {
+        T sa = sa0, sb = sb0;
+        if (stage_mask) {
+          const int ta = oa + n, tb = ob + n;
+          if ((ra[ta >> 5] >> (ta & 31)) & 1u) sa = (T)0;
+          if ((rb[tb >> 5] >> (tb & 31)) & 1u) sb = (T)0;
+        }
+        cpx<T> z = *q;
+        z.x *= sa;
+        z.y *= sb;
+        *q = z;
+      }
+    } else {
     staged<NT, K, Masked<T>>(
         totr,
         [&](int idx) {
@@ -841,6 +868,7 @@ __device__ __forceinline__ void dim1_body(const PassArgs<T>& a, int tile) {
             lds[n * ps + p] = z;
           }
         });
+    }
     if (MODE == MODE_C2R) return;
     __syncthreads();
   } else {
