@@ -55,10 +55,13 @@ def build(verbose=True, incremental=False):
     units = fft_instances.write(OBJ, NUNITS)
     headers = [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hpp")]
     headers.append(os.path.join(INC, "csipm.h"))
+    # the kernel units include csipm_kernels.hpp and what it includes
+    kheaders = [os.path.join(CSRC, f) for f in ("csipm_kernels.hpp", "csipm_device.hpp",
+                                                "radix_consts.hpp")]
     jobs = []
     for u in units:
         o = u[:-4] + ".o"
-        jobs.append((u, o, headers + [u]))
+        jobs.append((u, o, kheaders + [u]))
     main_o = os.path.join(OBJ, "csipm.o")
     jobs.append((os.path.join(CSRC, "csipm.hip"), main_o,
                  headers + [os.path.join(CSRC, "csipm.hip"), os.path.join(OBJ, "fft_extern.inc")]))

@@ -1230,6 +1230,15 @@ int engine_apply_timed(Engine* E, int64_t nrep, double* ms, double* bytes, std::
     bytes[k] = b;
     names[k] = std::to_string(sb.spec.D) + (sb.spec.kind >= PK_COL ? "d-4step:" : "d:") +
                nm[sb.pass];
+    // a pass split over several launches: each name carries its workgroup
+    // size and LDS bytes
+    int same = 0;
+    for (const ChainSub& o : c.subs)
+      if (o.pass == sb.pass && o.spec.D == sb.spec.D && o.spec.kind == sb.spec.kind &&
+          o.spec.axis == sb.spec.axis)
+        ++same;
+    if (same > 1)
+      names[k] += "@nt" + std::to_string(sb.nt) + "/lds" + std::to_string(sb.lds);
   }
   *nsub = ns;
   return CSIPM_OK;
