@@ -28,7 +28,8 @@ request arrives.
 * roofline: the dominant launch of the batched CG operator apply over the
   stream's first 512 requests admitted together, HIP events on the engine's
   stream (csipm_batch_apply_timed); `--roofline-only` runs only that part
-  (the command profiled with rocprofv3).
+  (the command profiled with rocprofv3); `--batched-only` times the
+  batched server alone (the stream's kernel-time profile).
 * cpu_baseline: the FP64 CPU restatement of the reference path (oracle/,
   scipy pocketfft) on the host cores: the 1D N=1024 configuration and a
   fixed sample of stream requests.
@@ -368,6 +369,15 @@ def run_stream(args):
     solved_b = sum(r.status == "solved" for r in out_b)
     log(f"batched mixed at saturation: {n_timed} requests in {t_b:.2f} s ({value:.1f}/s, "
         f"{solved_b} solved)")
+    if args.batched_only:
+        print(json.dumps({"metric": METRIC, "value": round(value, 3), "unit": "solves/s",
+                          "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+                          "ms_per_step": round(1e3 * t_b / args.steps, 3),
+                          "higher_is_better": True, "requests": n_timed,
+                          "solved": solved_b,
+                          "note": "batched arm only (--batched-only): no plain FP64 arm, "
+                                  "latency, roofline or CPU baseline"}), flush=True)
+        return 0
     res_p, _, t_p = plain(idx, None, cap_s=args.plain_cap_s)
     n_p = len(res_p)
     plain_rate = n_p / t_p
@@ -629,6 +639,8 @@ def main():
     ap.add_argument("--plain-cap-s", type=float, default=100.0,
                     help="stop the plain arm after this many seconds (reported)")
     ap.add_argument("--roofline-only", action="store_true")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="stream: time the batched server only (for profiling)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:

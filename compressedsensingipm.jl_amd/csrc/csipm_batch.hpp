@@ -421,7 +421,7 @@ __global__ __launch_bounds__(256) void bk_absmax(BSeg seg, const BInst* __restri
 
 __global__ __launch_bounds__(256) void bk_ipm_init(BSeg seg, const BInst* __restrict__ info,
                                                    BFlags* flags, double* beta, double* c,
-                                                   double* z1, double* z2,
+                                                   double* z1, double* z2, double* Qb,
                                                    const double* scal_all,
                                                    const uint32_t* valid) {
   BSEG_BEGIN
@@ -430,6 +430,7 @@ __global__ __launch_bounds__(256) void bk_ipm_init(BSeg seg, const BInst* __rest
   BFOR(i) {
     const bool ok = BVALID(valid, i);
     beta[i] = 0.0;
+    Qb[i] = 0.0;   // M^T M beta
     c[i] = ok ? cval : 1.0;
     z1[i] = ok ? zval : 0.0;
     z2[i] = ok ? zval : 0.0;
@@ -574,15 +575,23 @@ __global__ __launch_bounds__(256) void bk_convert(BSeg seg, const BInst* __restr
   }
 }
 
-// y += x (first block), x in the CG precision
+// the refinement's correction x (TC, reduced system) into the FP64 direction
+// y, and M^T M x = b - r - D x (b, r: that CG's right-hand side and final
+// recursive residual, D the reduced diagonal) into the second block of mtm,
+// whose first block the FP64 apply of the uncorrected y left M^T M y in
 template <typename TC>
-__global__ __launch_bounds__(256) void bk_widen_add(BSeg seg, const BInst* __restrict__ info,
-                                                    const BFlags* flags, double* y,
-                                                    const TC* __restrict__ x) {
+__global__ __launch_bounds__(256) void bk_refine_add(BSeg seg, const BInst* __restrict__ info,
+                                                     const BFlags* flags, double* y,
+                                                     const TC* __restrict__ x, double* mtm,
+                                                     const TC* __restrict__ b,
+                                                     const TC* __restrict__ r,
+                                                     const TC* __restrict__ D) {
   BSEG_BEGIN
   BFOR(i) {
     const int64_t j = i + I.off;
-    y[j] += (double)x[j];
+    const double xv = (double)x[j];
+    y[j] += xv;
+    mtm[j + I.n] += (double)b[j] - (double)r[j] - (double)D[i] * xv;
   }
 }
 
@@ -701,7 +710,8 @@ template <typename TD>
 __global__ __launch_bounds__(256) void bk_ipm_update(
     BSeg seg, const BInst* __restrict__ info, const BFlags* flags, double* beta, double* c,
     double* z1, double* z2, const TD* __restrict__ d, const double* __restrict__ dz1,
-    const double* __restrict__ dz2, const double* scal_all, const uint32_t* valid) {
+    const double* __restrict__ dz2, double* Qb, const double* __restrict__ mtm,
+    const double* scal_all, const uint32_t* valid) {
   BSEG_BEGIN
   const double* scal = scal_all + (int64_t)inst * S_NSLOTS;
   const double ap = scal[S_AP], ad = scal[S_AD];
@@ -709,6 +719,7 @@ __global__ __launch_bounds__(256) void bk_ipm_update(
     if (!BVALID(valid, i)) continue;
     const int64_t j = i + I.off;
     beta[i] += ap * (double)d[j];
+    if (Qb) Qb[i] += ap * mtm[j + I.n];   // Q beta carried with beta (M^T M d in mtm)
     c[i] += ap * (double)d[j + I.n];
     z1[i] += ad * dz1[i];
     z2[i] += ad * dz2[i];

@@ -537,6 +537,7 @@ int chain_call_args(Engine* E, int id, int call, int nb, const PassSpec& s,
       e.xb = (const T*)(E->W2(V2_DD) + o2);
       e.L1 = (const T*)(E->W(V_DGD) + off);
       e.yb = (T*)(E->W2(V2_TMP) + o2);
+      e.out = (T*)(E->W2(V2_TMP) + o2 + n);   // M^T M of the direction, for Q beta
       e.alpha = (T)-1;
       e.beta = (T)1;
       break;
@@ -758,8 +759,8 @@ int engine_tables(Engine* E, int nb, bool ipm) {
   CHK(build_chain<TC>(E, A, E->active, CALL_CGA, nb, E->chains[CALL_CGA]));
   CHK(build_chain<TC>(E, A, E->active, CALL_CGR, nb, E->chains[CALL_CGR]));
   if (ipm) {
-    CHK(build_chain<double>(E, A, E->active, CALL_QB, nb, E->chains[CALL_QB]));
     if (E->mixed) CHK(build_chain<double>(E, A, E->active, CALL_REF, nb, E->chains[CALL_REF]));
+    else CHK(build_chain<double>(E, A, E->active, CALL_QB, nb, E->chains[CALL_QB]));
   }
   char* base = nullptr;
   CHK(A.commit(E->stream, &base));
@@ -886,7 +887,8 @@ int engine_init_fresh(Engine* E) {
     if ((rc = finalize(E, seg, ids, (int)fresh.size(), {1}, 0, S_BNORM, 0, 0, 0)) != CSIPM_OK)
       break;
     hipLaunchKernelGGL(bk_ipm_init, dim3(grid), dim3(kNT), 0, E->stream, seg, E->info, E->flags,
-                       E->W(V_BETA), E->W(V_C), E->W(V_Z1), E->W(V_Z2), E->scal, E->valid);
+                       E->W(V_BETA), E->W(V_C), E->W(V_Z1), E->W(V_Z2), E->W(V_QB), E->scal,
+                       E->valid);
     if (hipGetLastError() != hipSuccess) {
       rc = fail(CSIPM_ERR_HIP, "init launch failed");
       break;
@@ -1038,8 +1040,9 @@ int engine_newton(Engine* E, double atol, double rtol, int reliable, bool refine
                        E->flags, E->W2(V2_TMP), E->C2<TC>(C2_RHS), 1);
     HIPCHK(hipGetLastError());
     CHK(engine_cg<TC>(E, 1, 0.0, kRefineRtol, reliable));
-    hipLaunchKernelGGL((bk_widen_add<TC>), dim3(E->vgrid), dim3(kNT), 0, st, sg, E->info,
-                       E->flags, E->W2(V2_DD), E->C2<TC>(C2_D));
+    hipLaunchKernelGGL((bk_refine_add<TC>), dim3(E->vgrid), dim3(kNT), 0, st, sg, E->info,
+                       E->flags, E->W2(V2_DD), E->C2<TC>(C2_D), E->W2(V2_TMP),
+                       E->C2<TC>(C2_RHS), E->C2<TC>(C2_R), E->C<TC>(C_L1));
   }
   hipLaunchKernelGGL(bk_red_back, dim3(E->vgrid), dim3(kNT), 0, st, sg, E->info, E->flags,
                      E->W2(V2_DD), E->W2(V2_RHSD), E->W(V_L1D), E->W(V_L2D), E->valid);
@@ -1077,7 +1080,12 @@ int engine_step(Engine* E, double tol, int max_iter, double cg_atol, double cg_r
   double *b = E->W(V_B), *beta = E->W(V_BETA), *c = E->W(V_C), *z1 = E->W(V_Z1);
   double *z2 = E->W(V_Z2), *Qb = E->W(V_QB), *s1 = E->W(V_S1), *s2 = E->W(V_S2);
   double *dt1 = E->W(V_DT1), *dt2 = E->W(V_DT2), *dz1 = E->W(V_DZ1), *dz2 = E->W(V_DZ2);
-  CHK(run_chain<double>(E->chains[CALL_QB], st));
+  // Q beta = M^T M beta: an FP64 apply per step in FP64; mixed, it is carried
+  // from beta = 0 by the updates beta += alpha d, Q beta += alpha M^T M d,
+  // M^T M d from the corrector's FP64 refinement apply (its uncorrected
+  // part) and its refinement CG (the correction's part, b - r - D x in FP32:
+  // the correction is a 1e-5 part of d, so FP64-level accuracy)
+  if (!E->mixed) CHK(run_chain<double>(E->chains[CALL_QB], st));
   hipLaunchKernelGGL(bk_ipm_resid, dim3(g), dim3(kNT), 0, st, sg, E->info, E->flags, beta, c, z1,
                      z2, Qb, b, E->part, E->valid);
   CHK(finalize(E, sg, E->ids, na, {1, 1, 0, 0, 0, 0}, 0, S_RB, tau, tol, max_iter));
@@ -1128,10 +1136,12 @@ int engine_step(Engine* E, double tol, int max_iter, double cg_atol, double cg_r
   CHK(finalize(E, sg, E->ids, na, {2, 2}, 3, 0, tau, tol, max_iter));
   if (E->mixed)
     hipLaunchKernelGGL((bk_ipm_update<double>), dim3(g), dim3(kNT), 0, st, sg, E->info, E->flags,
-                       beta, c, z1, z2, E->W2(V2_DD), dz1, dz2, E->scal, E->valid);
+                       beta, c, z1, z2, E->W2(V2_DD), dz1, dz2, Qb, E->W2(V2_TMP), E->scal,
+                       E->valid);
   else
     hipLaunchKernelGGL((bk_ipm_update<TC>), dim3(g), dim3(kNT), 0, st, sg, E->info, E->flags,
-                       beta, c, z1, z2, E->C2<TC>(C2_D), dz1, dz2, E->scal, E->valid);
+                       beta, c, z1, z2, E->C2<TC>(C2_D), dz1, dz2, nullptr, nullptr, E->scal,
+                       E->valid);
   HIPCHK(hipGetLastError());
   return CSIPM_OK;
 }

@@ -179,7 +179,7 @@ template <typename T>
 struct Epi {
   int kind;             // 0: out[i] = value          1: condensed KKT epilogue
                         // 2: reduced (Schur) epilogue y = alpha (m + D x) + beta y, D in L1
-  T* out;
+  T* out;               // kind 0: the value; kind 2: m itself as well, when not null
   const T* xb;
   const T* xz;
   const T* L1;
@@ -198,6 +198,7 @@ __device__ __forceinline__ double epi_apply(const Epi<T>& e, int64_t i, T m) {
     return 0.0;
   }
   if (e.kind == 2) {
+    if (e.out) e.out[i] = m;
     const T xb = e.xb[i];
     T qb = e.alpha * (m + e.L1[i] * xb);
     if (e.beta != (T)0) qb += e.beta * e.yb[i];
@@ -224,6 +225,7 @@ __device__ __forceinline__ double epi_apply_u(const Epi<T>& e, int64_t el, cpx<T
     return 0.0;
   }
   if (e.kind == 2) {
+    if (e.out) reinterpret_cast<cpx<T>*>(e.out)[el] = m;
     const cpx<T> xb = reinterpret_cast<const cpx<T>*>(e.xb)[el];
     const cpx<T> dg = reinterpret_cast<const cpx<T>*>(e.L1)[el];
     cpx<T> qb = {e.alpha * (m.x + dg.x * xb.x), e.alpha * (m.y + dg.y * xb.y)};
@@ -291,6 +293,7 @@ __device__ __forceinline__ double epi_commit_u(const Epi<T>& e, int64_t el, cpx<
     return 0.0;
   }
   if (e.kind == 2) {
+    if (e.out) reinterpret_cast<cpx<T>*>(e.out)[el] = m;
     cpx<T> qb = {e.alpha * (m.x + o.l1.x * o.xb.x), e.alpha * (m.y + o.l1.y * o.xb.y)};
     if (e.beta != (T)0) {
       qb.x += e.beta * o.ob.x;
