@@ -331,7 +331,9 @@ def run_stream(args):
         out = serve(reqs, lambda q: make_mixed(q.p), mixed=True, budget_bytes=BUDGET_BYTES,
                     max_instances=args.max_instances, tol=STREAM_TOL,
                     prep_workers=args.prep_workers, on_done=progress, stats=st, solver=server)
+        st["returned_s"] = time.perf_counter() - t_s
         torch.cuda.synchronize()
+        st["synced_s"] = time.perf_counter() - t_s
         log("serve accounting: " + json.dumps({k: round(v, 3) for k, v in st.items()}))
         return out, time.perf_counter() - t_s
 

@@ -72,6 +72,10 @@ SIGNATURES = {
     "csipm_batch_step": (_i, [_vp, _d, _i64, _d, _d, ctypes.POINTER(_i)]),
     "csipm_batch_poll": (_i, [_vp, ctypes.POINTER(_i), _i, ctypes.POINTER(_i)]),
     "csipm_batch_harvest": (_i, [_vp, _i, ctypes.POINTER(_d)]),
+    "csipm_batch_admit_many": (_i, [_vp, _i, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                    ctypes.POINTER(_d), ctypes.POINTER(_vp), _i64,
+                                    ctypes.POINTER(_i), ctypes.POINTER(_i)]),
+    "csipm_batch_harvest_many": (_i, [_vp, _i, ctypes.POINTER(_i), ctypes.POINTER(_d)]),
     "csipm_batch_apply_timed": (_i, [_vp, _i64, ctypes.POINTER(_d), ctypes.POINTER(_d), _i,
                                      ctypes.POINTER(_i)]),
     "csipm_batch_launch_name": (ctypes.c_char_p, [_vp, _i]),

@@ -13,6 +13,7 @@ from __future__ import annotations
 
 import ctypes
 import heapq
+import itertools
 import time
 from dataclasses import dataclass, field
 
@@ -125,6 +126,53 @@ class BatchSolver:
         self._live[i.value] = (op, z, beta, tag, time.perf_counter())
         return i.value
 
+    def admit_many(self, items, cg_itmax=0):
+        """Admit several problems in one call (csipm_batch_admit_many).
+        items: (op, z, beta, lam, tag) with z and beta FP64 device vectors of
+        op.N entries (z0 and the result buffer, e.g. from `buffers`).
+        Returns one entry per item: its id, None when it does not fit now
+        (later, smaller ones are still tried), or the CsipmError that
+        admitting it raised (the others are unaffected)."""
+        n = len(items)
+        if n == 0:
+            return []
+        out = [None] * n
+        ok = []
+        for k, (op, *_) in enumerate(items):
+            if op.mixed != self.mixed:
+                out[k] = ValueError("operator precision differs from the batch")
+            else:
+                ok.append(k)
+        m = len(ok)
+        if m == 0:
+            return out
+        hs = (ctypes.c_void_p * m)(*[items[k][0].handle for k in ok])
+        zs = (ctypes.c_void_p * m)(*[_ptr(items[k][1]) for k in ok])
+        bv = (ctypes.c_void_p * m)(*[_ptr(items[k][2]) for k in ok])
+        lams = (ctypes.c_double * m)(*[float(items[k][3]) for k in ok])
+        ids = (ctypes.c_int * m)()
+        rcs = (ctypes.c_int * m)()
+        rc = _lib.lib().csipm_batch_admit_many(self._b, m, hs, zs, lams, bv, int(cg_itmax), ids,
+                                               rcs)
+        t = time.perf_counter()
+        for q, k in enumerate(ok):
+            op, z, beta, lam, tag = items[k]
+            if ids[q] >= 0:
+                self._live[ids[q]] = (op, z, beta, tag, t)
+                out[k] = ids[q]
+            elif rcs[q] not in (_lib.OK, _lib.ERR_ALLOC):
+                out[k] = _lib.CsipmError(rcs[q], f"admission of problem {k} failed "
+                                                 f"(code {rcs[q]})")
+        _lib.check(rc)
+        return out
+
+    @staticmethod
+    def buffers(op, z0):
+        """(z, beta): z0 as an FP64 device vector and an empty result vector,
+        the per-problem buffers admit_many takes."""
+        return (op._vec(z0, dtype=torch.float64),
+                torch.empty(op.N, dtype=torch.float64, device=op.device))
+
     def step(self, tol=1e-8, max_iter=200, cg_rtol=1e-10, cg_atol=0.0):
         """One interior-point iteration of every admitted, unfinished
         instance; returns the number still iterating.  cg_atol / cg_rtol: the
@@ -153,6 +201,26 @@ class BatchSolver:
             retire.append(op)
         return Result(beta, info[0], int(info[1]), int(info[2]), _STATUS[int(info[3])], info[4],
                       info[5], tag, t0, time.perf_counter())
+
+    def harvest_many(self, ids, retire=None):
+        """harvest() of several finished instances in one call
+        (csipm_batch_harvest_many)."""
+        n = len(ids)
+        if n == 0:
+            return []
+        idv = (ctypes.c_int * n)(*ids)
+        info = (ctypes.c_double * (8 * n))()
+        _lib.call("csipm_batch_harvest_many", self._b, n, idv, info)
+        t = time.perf_counter()
+        out = []
+        for k, i in enumerate(ids):
+            op, z, beta, tag, t0 = self._live.pop(i)
+            if retire is not None:
+                retire.append(op)
+            f = info[8 * k:8 * k + 8]
+            out.append(Result(beta, f[0], int(f[1]), int(f[2]), _STATUS[int(f[3])], f[4], f[5],
+                              tag, t0, t))
+        return out
 
     @property
     def live(self):
@@ -253,14 +321,24 @@ def serve(requests, make_op, mixed=True, budget_bytes=200 << 30, pool_fraction=0
     pending = list(requests)
     heapq.heapify(pending)
     waiting = deque()   # arrived, not admitted: [req, future or op, bytes held]
+    nsub = 0            # waiting requests (a prefix) whose handles are made or in the making
     out = []
     held = 0            # handle bytes of waiting (prepared) and live requests
     live_bytes = {}     # request seq -> handle bytes
     dev = torch.cuda.current_device()
 
     def prepare(problem):
+        # the handle and the request's device buffers (z0 in FP64, beta),
+        # made off the loop's critical path
         with torch.cuda.device(dev):
-            return make_op(problem)
+            op = make_op(problem)
+            try:
+                z, beta = BatchSolver.buffers(op, problem.z0)
+                nb = op_bytes(op)
+            except Exception:
+                op.close()
+                raise
+            return op, z, beta, nb
 
     def finish(r):
         out.append(r)
@@ -289,56 +367,72 @@ def serve(requests, make_op, mixed=True, budget_bytes=200 << 30, pool_fraction=0
                                           f"{hb} handle bytes (share {handle_budget})", now))
                     continue
                 waiting.append([req, None, hb])
-            # make handles, in arrival order, while their bytes fit
-            for w in waiting:
-                if w[1] is not None:
-                    continue
+            # make handles, in arrival order, while their bytes fit (the
+            # first nsub waiting requests have theirs made or in the making)
+            for w in itertools.islice(waiting, nsub, None):
                 if held + w[2] > handle_budget:
                     break
                 held += w[2]
+                nsub += 1
                 w[1] = prep.submit(prepare, w[0].problem) if prep is not None else \
                     _made(prepare, w[0].problem)
-            # admit ready requests in arrival order
+            # admit ready requests in arrival order, in one call
             t_a = time.perf_counter()
             skipped = 0
-            admitted = 0
-            k = 0
-            while k < len(waiting) and skipped <= scan:
-                w = waiting[k]
-                req, f = w[0], w[1]
-                if f is None:
+            scanned = 0
+            cand = []          # (waiting entry, op, z, beta, bytes)
+            room = bs.max_instances - bs.live + scan   # more could not be admitted
+            for w in waiting:
+                if w[1] is None or skipped > scan or len(cand) >= room:
                     break
-                if not f.done():
+                scanned += 1
+                if not w[1].done():
                     skipped += 1
-                    k += 1
                     continue
                 try:
-                    op = f.result()
-                    nb = op_bytes(op)
+                    op, z, beta, nb = w[1].result()
+                except (_lib.CsipmError, ValueError, RuntimeError) as e:
+                    w[1] = _Ready(None, e)
+                    cand.append((w, None, None, None, 0))
+                    continue
+                cand.append((w, op, z, beta, nb))
+            todo = [c for c in cand if c[1] is not None]
+            got = bs.admit_many([(op, z, beta, w[0].problem.lam, w[0])
+                                 for w, op, z, beta, _ in todo])
+            res = {id(c[0]): g for c, g in zip(todo, got)}
+            gone = set()
+            for w, op, z, beta, nb in cand:
+                req = w[0]
+                if op is not None:
                     held += nb - w[2]
                     w[2] = nb
-                    i = bs.admit(op, req.problem.z0, req.problem.lam, tag=req)
-                except (_lib.CsipmError, ValueError, RuntimeError) as e:
+                g = res.get(id(w)) if op is not None else w[1].exc
+                if isinstance(g, Exception):
                     held -= w[2]
-                    del waiting[k]
-                    finish(_rejected(req, str(e), clock() - t_start))
-                    continue
-                if i is None:
+                    gone.add(id(w))
+                    if op is not None:
+                        op.close()
+                    finish(_rejected(req, str(g), clock() - t_start))
+                elif g is None:
                     if bs.live == 0:
                         # an empty engine cannot take it: it never will
                         held -= w[2]
-                        del waiting[k]
+                        gone.add(id(w))
                         op.close()
                         finish(_rejected(req, "does not fit an empty engine",
                                          clock() - t_start))
-                        continue
-                    w[1] = _Ready(op)
-                    skipped += 1
-                    k += 1
-                    continue
-                live_bytes[req.seq] = w[2]
-                del waiting[k]
-                admitted += 1
+                    else:
+                        w[1] = _Ready((op, z, beta, nb))
+                else:
+                    live_bytes[req.seq] = w[2]
+                    gone.add(id(w))
+            if gone:
+                # the removed requests are among the first `scanned`
+                head = [waiting.popleft() for _ in range(scanned)]
+                for w in reversed(head):
+                    if id(w) not in gone:
+                        waiting.appendleft(w)
+                nsub -= len(gone)
             acc["admit_s"] += time.perf_counter() - t_a
             if bs.live == 0:
                 t_i = time.perf_counter()
@@ -358,8 +452,7 @@ def serve(requests, make_op, mixed=True, budget_bytes=200 << 30, pool_fraction=0
             acc["step_s"] += t_h - t_s
             acc["steps"] += 1
             retired = []
-            for i in bs.finished():
-                r = bs.harvest(i, retire=retired)
+            for r in bs.harvest_many(bs.finished(), retire=retired):
                 r.t_admit -= t_start
                 r.t_done = clock() - t_start
                 held -= live_bytes.pop(r.tag.seq, 0)
@@ -380,10 +473,13 @@ def serve(requests, make_op, mixed=True, budget_bytes=200 << 30, pool_fraction=0
                 pass
             stats.update({"loop_" + k: v for k, v in acc.items()})
             stats["wall_s"] = clock() - t_start
+        t_c = time.perf_counter()
         if solver is None:
             bs.close()
         if prep is not None:
             prep.shutdown(wait=True, cancel_futures=True)
+        if stats is not None:
+            stats["shutdown_s"] = time.perf_counter() - t_c
     return out
 
 

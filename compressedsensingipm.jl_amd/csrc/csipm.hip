@@ -11,6 +11,7 @@
 #include <initializer_list>
 #include <limits>
 #include <map>
+#include <unordered_set>
 #include <memory>
 #include <mutex>
 #include <unordered_map>
@@ -2587,6 +2588,43 @@ int csipm_batch_admit(csipm_batch_t b, csipm_op_t h, const double* z0, double la
   if (!(lambda >= 0)) return fail(CSIPM_ERR_ARG, "bad lambda");
   CHK(engine_enter(b->E));
   return engine_admit(b->E, h, z0, lambda, beta_out, cg_itmax, id);
+}
+
+int csipm_batch_admit_many(csipm_batch_t b, int n, const csipm_op_t* ops,
+                           const double* const* z0, const double* lambda,
+                           double* const* beta_out, int64_t cg_itmax, int* ids, int* rcs) {
+  if (!b || n < 0 || (n > 0 && (!ops || !z0 || !lambda || !beta_out || !ids || !rcs)))
+    return fail(CSIPM_ERR_ARG, "null argument");
+  CHK(engine_enter(b->E));
+  std::unordered_set<csipm_op_t> seen;
+  for (int k = 0; k < n; ++k) {
+    ids[k] = -1;
+    int rc = CSIPM_OK;
+    if (!ops[k] || !z0[k] || !beta_out[k]) rc = fail(CSIPM_ERR_ARG, "null argument %d", k);
+    else if (ops[k]->prec != b->prec)
+      rc = fail(CSIPM_ERR_ARG, "handle %d: precision differs from the batch", k);
+    else if (!(lambda[k] >= 0)) rc = fail(CSIPM_ERR_ARG, "bad lambda %d", k);
+    else if (!seen.insert(ops[k]).second) rc = fail(CSIPM_ERR_ARG, "handle %d appears twice", k);
+    else rc = engine_admit(b->E, ops[k], z0[k], lambda[k], beta_out[k], cg_itmax, &ids[k]);
+    rcs[k] = rc;
+    if (rc == CSIPM_ERR_HIP) return rc;
+  }
+  return CSIPM_OK;
+}
+
+int csipm_batch_harvest_many(csipm_batch_t b, int n, const int* ids, double* info) {
+  if (!b || n < 0 || (n > 0 && (!ids || !info))) return fail(CSIPM_ERR_ARG, "null argument");
+  Engine* E = b->E;
+  std::vector<char> seen((size_t)E->maxinst, 0);
+  for (int k = 0; k < n; ++k) {
+    const int id = ids[k];
+    if (id < 0 || id >= E->maxinst || !E->inst[(size_t)id].h || E->inst[(size_t)id].fresh)
+      return fail(CSIPM_ERR_ARG, "no admitted instance %d", id);
+    if (seen[(size_t)id]++) return fail(CSIPM_ERR_ARG, "instance %d appears twice", id);
+  }
+  CHK(engine_enter(E));
+  for (int k = 0; k < n; ++k) CHK(engine_harvest(E, ids[k], info + 8 * (size_t)k, false));
+  return engine_leave(E);
 }
 
 int csipm_batch_step(csipm_batch_t b, double tol, int64_t max_iter, double cg_atol,

@@ -362,6 +362,18 @@ int csipm_batch_poll(csipm_batch_t b, int* ids, int cap, int* n);
  * written asynchronously, in order on the batch's stream (the host does not
  * wait); info comes from the flags the last step read.                   */
 int csipm_batch_harvest(csipm_batch_t b, int id, double* info);
+/* The same for n problems / instances in one call (a serving loop's turn):
+ * one stream hand-over for all of them instead of one per call.
+ * admit_many: rcs[k] = what csipm_batch_admit would return for problem k
+ * (CSIPM_ERR_ALLOC: it does not fit now; later, smaller problems are still
+ * tried; CSIPM_ERR_ARG also when a handle appears twice), ids[k] its id or
+ * -1.  Returns CSIPM_OK unless a HIP call fails (problems before it stay
+ * admitted).  harvest_many: every id must name a finished instance (checked
+ * before any is harvested); info: n * 8 doubles.                         */
+int csipm_batch_admit_many(csipm_batch_t b, int n, const csipm_op_t* ops,
+                           const double* const* z0, const double* lambda,
+                           double* const* beta_out, int64_t cg_itmax, int* ids, int* rcs);
+int csipm_batch_harvest_many(csipm_batch_t b, int n, const int* ids, double* info);
 /* Convenience: admit n problems, iterate until all finish, harvest all.
  * info: n * 8 doubles.                                                   */
 int csipm_batch_solve(csipm_batch_t b, int n, const csipm_op_t* ops,

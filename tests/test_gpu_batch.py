@@ -7,7 +7,8 @@
   FFT passes differently, so the CG dot products round differently.)
 * Admission rules: a handle serves one live instance (admitting it twice
   raises); serve() returns a request that can never fit its budget as
-  "rejected" at once and solves the others.
+  "rejected" at once and solves the others.  The batched admission /
+  harvest calls give per problem what the single ones give.
 * Failure statuses forced from the host: an IPM iteration cap, a CG
   iteration cap of one, a non-finite sample in z0.
 * Continuous batching: problems admitted while others are mid-solve, and
@@ -102,6 +103,48 @@ def test_continuous_batching(cuda):
     assert sorted(done) == list(range(len(probs)))
     for k, p in enumerate(probs):
         _same(done[k], _alone(p, True, 1e-7), p.name)
+
+
+def test_admit_harvest_many(cuda):
+    """The batched admission / harvest calls behave as the single ones: per
+    problem, an id, None when it does not fit (a later, smaller one is still
+    admitted) or the error (a handle given twice); results equal the alone
+    solves."""
+    from csipm import _lib
+    from csipm.batch import BatchSolver, slots_of
+    print("admit / harvest many", flush=True)
+    probs = _problems()
+    ops = [_op(p, True) for p in probs]
+    sl = [slots_of(o) for o in ops]
+    big = int(np.argmax(sl))
+    small = [k for k in range(len(probs)) if k != big]
+    # room for every problem but the largest; admitted largest-first except
+    # the smallest, the largest then finds only the smallest's room left and
+    # does not fit, the smallest after it does; one handle is given twice
+    small.sort(key=lambda k: -sl[k])
+    bs = BatchSolver(sum(sl[k] for k in small), max_instances=len(probs), mixed=True)
+    order = small[:-1] + [big, small[-1], small[0]]
+    items = []
+    for k in order:
+        z, beta = BatchSolver.buffers(ops[k], probs[k].z0)
+        items.append((ops[k], z, beta, probs[k].lam, k))
+    got = bs.admit_many(items)
+    assert all(isinstance(g, int) for g in got[:len(small) - 1]), got
+    assert got[-3] is None, got
+    assert isinstance(got[-2], int), got
+    assert isinstance(got[-1], _lib.CsipmError) and got[-1].code == _lib.ERR_ARG, got
+    assert bs.live == len(small)
+    done = {}
+    while bs.live:
+        bs.step(tol=1e-7)
+        for r in bs.harvest_many(bs.finished()):
+            done[r.tag] = r
+    with pytest.raises(_lib.CsipmError):
+        bs.harvest_many([0])             # nothing admitted under that id any more
+    bs.close()
+    assert sorted(done) == sorted(small)
+    for k in small:
+        _same(done[k], _alone(probs[k], True, 1e-7), probs[k].name)
 
 
 def test_mixed_batch_vs_oracle(cuda):
