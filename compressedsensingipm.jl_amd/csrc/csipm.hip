@@ -10,7 +10,9 @@
 #include <cstring>
 #include <initializer_list>
 #include <limits>
+#include <deque>
 #include <map>
+#include <set>
 #include <unordered_set>
 #include <memory>
 #include <mutex>
@@ -94,12 +96,15 @@ static std::mutex g_capture_mu;
 // a handle per request reuses blocks instead of calling hipMalloc / hipFree
 // (which synchronises the whole device) for each.  Blocks above
 // kCacheMaxBlock bypass the cache, at most kCacheKeepBytes per device stay
-// cached, and an allocation that fails empties the cache and retries.  The
-// owners make sure no work that uses a block is in flight when they return
-// it (handles and engines synchronise their stream first).
+// cached (a serving stream's handles churn tens of GB: a smaller cache
+// overflows into a device-synchronising hipFree per block), and an
+// allocation that fails empties the cache and retries.  The owners make sure
+// no work that uses a block is in flight when they return it (engines
+// synchronise their stream; handles release their blocks once an event on
+// theirs has completed).
 // ---------------------------------------------------------------------------
 constexpr size_t kCacheMaxBlock = size_t(1) << 30;
-constexpr size_t kCacheKeepBytes = size_t(4) << 30;
+constexpr size_t kCacheKeepBytes = size_t(32) << 30;
 
 struct BlockCache {
   std::mutex mu;
@@ -229,6 +234,57 @@ static void dfree(void* p) {
     }
   }
   (void)hipFree(p);
+}
+
+// dfree for callers that hold no capture lock: only a block the cache does
+// not keep is freed, and only that takes the lock (hipFree must not run
+// beside another thread's graph capture)
+static void dfree_release(void* p) {
+  if (!p) return;
+  {
+    std::lock_guard<std::mutex> lk(g_blocks.mu);
+    auto it = g_blocks.owner.find(p);
+    if (it != g_blocks.owner.end()) {
+      const int64_t key = it->second;
+      const int dev = (int)(key >> 8);
+      const size_t sz = (size_t)1 << (key & 255);
+      if (g_blocks.cached[dev] + sz <= kCacheKeepBytes) {
+        g_blocks.free[key].push_back(p);
+        g_blocks.cached[dev] += sz;
+        return;
+      }
+      g_blocks.owner.erase(it);
+    }
+  }
+  std::lock_guard<std::mutex> lk(g_capture_mu);
+  (void)hipFree(p);
+}
+
+// Blocks of destroyed handles wait here until the work queued on their
+// handle's stream before the destroy has finished (an event recorded at the
+// destroy), then go back to the block cache: a destroy never waits for a
+// stream (on the legacy stream that would also be every other thread's
+// queued work).  Ready entries are released by later destroys and creates.
+struct PendingBlocks {
+  hipEvent_t ev;
+  std::vector<void*> blocks;
+};
+static std::mutex g_pend_mu;
+static std::deque<PendingBlocks> g_pend;
+
+static void release_pending() {
+  std::vector<PendingBlocks> ready;
+  {
+    std::lock_guard<std::mutex> lk(g_pend_mu);
+    while (!g_pend.empty() && hipEventQuery(g_pend.front().ev) != hipErrorNotReady) {
+      ready.push_back(std::move(g_pend.front()));
+      g_pend.pop_front();
+    }
+  }
+  for (PendingBlocks& pb : ready) {
+    (void)hipEventDestroy(pb.ev);
+    for (void* b : pb.blocks) dfree_release(b);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -656,7 +712,9 @@ static int upload_rader(int p, void** bfw, void** binv) {
 // exp(-2 pi i t / L), t < L, in the op precision.  Tables are shared by the
 // handles on a device and counted: a table no handle holds stays cached
 // (a serving process meets the same lengths again) until the unheld tables
-// exceed kTwiddleKeepBytes, when the least recently used are freed.
+// exceed kTwiddleKeepBytes, when the least recently used go back to the
+// block cache.  Lookups by key and by pointer and the choice of a victim do
+// not scan the cache (a serving stream leaves thousands of lengths in it).
 constexpr size_t kTwiddleKeepBytes = size_t(256) << 20;
 
 struct TwiddleEntry {
@@ -667,7 +725,10 @@ struct TwiddleEntry {
 };
 
 static std::mutex g_tw_mu;
-static std::unordered_map<int64_t, TwiddleEntry> g_tw;
+static std::unordered_map<int64_t, TwiddleEntry> g_tw;      // key -> table
+static std::unordered_map<void*, int64_t> g_tw_key;         // table -> key
+static std::set<std::pair<uint64_t, int64_t>> g_tw_unheld;  // (last use, key), refs == 0
+static size_t g_tw_unheld_bytes = 0;
 static uint64_t g_tw_clock = 0;
 
 template <typename T>
@@ -675,13 +736,21 @@ static int upload_twiddles(int L, void** out) {
   int dev = 0;
   HIPCHK(hipGetDevice(&dev));
   const int64_t key = ((int64_t)dev << 40) | ((int64_t)sizeof(T) << 32) | (int64_t)L;
+  // an existing table with no holder is in the unheld set (a new one is not)
+  auto hold = [&](TwiddleEntry& e, bool fresh) {
+    if (e.refs == 0 && !fresh) {
+      g_tw_unheld.erase({e.last, key});
+      g_tw_unheld_bytes -= e.bytes;
+    }
+    e.refs += 1;
+    e.last = ++g_tw_clock;
+    *out = e.ptr;
+  };
   {
     std::lock_guard<std::mutex> lock(g_tw_mu);
     auto it = g_tw.find(key);
     if (it != g_tw.end()) {
-      it->second.refs += 1;
-      it->second.last = ++g_tw_clock;
-      *out = it->second.ptr;
+      hold(it->second, false);
       return CSIPM_OK;
     }
   }
@@ -693,10 +762,10 @@ static int upload_twiddles(int L, void** out) {
   void* p = nullptr;
   {
     std::lock_guard<std::mutex> lk(g_capture_mu);
-    HIPCHK(hipMalloc(&p, sizeof(cpx<T>) * (size_t)L));
+    HIPCHK(dmalloc(&p, sizeof(cpx<T>) * (size_t)L));
     const hipError_t e = hipMemcpy(p, h.data(), sizeof(cpx<T>) * (size_t)L, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
-      (void)hipFree(p);
+      dfree(p);
       return fail(CSIPM_ERR_HIP, "twiddle upload: %s", hipGetErrorString(e));
     }
   }
@@ -704,20 +773,17 @@ static int upload_twiddles(int L, void** out) {
   {
     std::lock_guard<std::mutex> lock(g_tw_mu);
     TwiddleEntry& e = g_tw[key];
-    if (e.ptr) {
+    const bool fresh = e.ptr == nullptr;
+    if (!fresh) {
       dup = p;            // another thread made the same table meanwhile
     } else {
       e.ptr = p;
       e.bytes = sizeof(cpx<T>) * (size_t)L;
+      g_tw_key[p] = key;
     }
-    e.refs += 1;
-    e.last = ++g_tw_clock;
-    *out = e.ptr;
+    hold(e, fresh);
   }
-  if (dup) {
-    std::lock_guard<std::mutex> lk(g_capture_mu);
-    (void)hipFree(dup);
-  }
+  dfree_release(dup);
   return CSIPM_OK;
 }
 
@@ -727,30 +793,24 @@ static void release_twiddles(void* p) {
   std::vector<void*> victims;
   {
     std::lock_guard<std::mutex> lock(g_tw_mu);
-    size_t unheld = 0;
-    for (auto& kv : g_tw) {
-      if (kv.second.ptr == p && kv.second.refs > 0) {
-        kv.second.refs -= 1;
-        p = nullptr;   // one reference per call
-      }
-      if (kv.second.refs == 0) unheld += kv.second.bytes;
+    auto k = g_tw_key.find(p);
+    if (k == g_tw_key.end()) return;
+    TwiddleEntry& e = g_tw[k->second];
+    if (e.refs > 0 && --e.refs == 0) {
+      g_tw_unheld.insert({e.last, k->second});
+      g_tw_unheld_bytes += e.bytes;
     }
-    while (unheld > kTwiddleKeepBytes) {
-      auto victim = g_tw.end();
-      for (auto it = g_tw.begin(); it != g_tw.end(); ++it)
-        if (it->second.refs == 0 &&
-            (victim == g_tw.end() || it->second.last < victim->second.last))
-          victim = it;
-      if (victim == g_tw.end()) break;
-      unheld -= victim->second.bytes;
-      victims.push_back(victim->second.ptr);
-      g_tw.erase(victim);
+    while (g_tw_unheld_bytes > kTwiddleKeepBytes && !g_tw_unheld.empty()) {
+      const int64_t vk = g_tw_unheld.begin()->second;
+      g_tw_unheld.erase(g_tw_unheld.begin());
+      auto v = g_tw.find(vk);
+      g_tw_unheld_bytes -= v->second.bytes;
+      victims.push_back(v->second.ptr);
+      g_tw_key.erase(v->second.ptr);
+      g_tw.erase(v);
     }
   }
-  if (!victims.empty()) {
-    std::lock_guard<std::mutex> lk(g_capture_mu);
-    for (void* v : victims) (void)hipFree(v);
-  }
+  for (void* v : victims) dfree_release(v);
 }
 
 // Launch-shape choices of the batched passes measured by the autotuner
@@ -1686,16 +1746,38 @@ static void destroy_impl(csipm_op_t h) {
   }
   // work of this handle still in flight on its stream finishes before its
   // blocks go back to the cache (engines synchronise their own stream)
-  (void)hipStreamSynchronize(h->stream);
-  std::lock_guard<std::mutex> lk(g_capture_mu);
-  delete h->solo;
+  if (h->solo) {
+    std::lock_guard<std::mutex> lk(g_capture_mu);
+    delete h->solo;   // synchronises its own stream
+  }
+  std::vector<void*> blocks;
   for (PrecState* ps : {&h->p64, &h->p32})
     for (int i = 0; i < 3; ++i)
-      for (void* b : {ps->chirp[i], ps->bfw[i], ps->binv[i]}) dfree(b);
-  for (int i = 0; i < 3; ++i) dfree(h->rperm[i]);
-  void* bufs[] = {h->miss, h->valid, h->buf, h->part, h->scratch, h->gz, h->dplan};
-  for (void* b : bufs) dfree(b);
+      for (void* b : {ps->chirp[i], ps->bfw[i], ps->binv[i]})
+        if (b) blocks.push_back(b);
+  for (int i = 0; i < 3; ++i)
+    if (h->rperm[i]) blocks.push_back(h->rperm[i]);
+  for (void* b : {(void*)h->miss, (void*)h->valid, h->buf, (void*)h->part, h->scratch,
+                  (void*)h->gz, (void*)h->dplan})
+    if (b) blocks.push_back(b);
+  // work of this handle still in flight on its stream finishes before its
+  // blocks go back to the cache (engines synchronise their own stream): at
+  // once when the stream is idle, else once an event recorded now completes
+  hipEvent_t ev = nullptr;
+  if (hipStreamQuery(h->stream) == hipSuccess) {
+    for (void* b : blocks) dfree_release(b);
+  } else if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess &&
+             hipEventRecord(ev, h->stream) == hipSuccess) {
+    std::lock_guard<std::mutex> lk(g_pend_mu);
+    g_pend.push_back({ev, std::move(blocks)});
+  } else {
+    if (ev) (void)hipEventDestroy(ev);
+    (void)hipGetLastError();
+    (void)hipStreamSynchronize(h->stream);
+    for (void* b : blocks) dfree_release(b);
+  }
   delete h;
+  release_pending();
 }
 
 // 1D lines above this length run as four-step transforms
@@ -1851,6 +1933,7 @@ static int create_impl(csipm_op* h, int dim, const int64_t sizes[3], int rdft, i
   CHK(plan_impl(h, dim, sizes, rdft, precision));
   h->stream = (hipStream_t)stream;
   CHK(no_capture(h->stream, "csipm_op_create"));
+  release_pending();   // blocks of destroyed handles whose work has finished
   if (n_missing < 0) return fail(CSIPM_ERR_ARG, "n_missing < 0");
   if (n_missing > 0 && !missing) return fail(CSIPM_ERR_ARG, "missing is null");
   if (h->big) CHK(upload_big1d(h));
