@@ -131,6 +131,8 @@ def main():
     ap.add_argument("--elems", type=float, default=2e7,
                     help="instances per shape: about this many samples per batch")
     ap.add_argument("--part", default="1/1", help="k/n: tune the k-th of n slices of the shapes")
+    ap.add_argument("--dims", type=int, default=0,
+                    help="tune only the shapes of this dimensionality (2 or 3)")
     ap.add_argument("--merge", nargs="*", default=None,
                     help="merge these partial tables into --out (no tuning)")
     args = ap.parse_args()
@@ -146,7 +148,8 @@ def main():
     entries = []
     t0 = time.perf_counter()
     k, n = (int(v) for v in args.part.split("/"))
-    for dims in list(dict.fromkeys(shapes(args.quick)))[k - 1::n]:
+    todo = [d for d in dict.fromkeys(shapes(args.quick)) if not args.dims or len(d) == args.dims]
+    for dims in todo[k - 1::n]:
         count = int(max(1, min(256, args.elems // int(np.prod(dims)))))
         entries += tune_shape(dims, count, args.reps, lib, log)
     # one entry per key (shapes sharing a key keep the larger gain)
@@ -157,7 +160,8 @@ def main():
         if k not in best or g > best[k]["ms_builtin"] / best[k]["ms_tuned"]:
             best[k] = e
     tab = {"generated_by": "python -m csipm.tune" + (" --quick" if args.quick else "") +
-                           f" --part {args.part}",
+                           f" --part {args.part}" + (f" --dims {args.dims}" if args.dims else "") +
+                           f" --elems {args.elems:g}",
            "device": _device_name(), "seconds": round(time.perf_counter() - t0, 1),
            "entries": sorted(best.values(), key=lambda e: (e["kind"], e["fft_len"], e["rows"]))}
     with open(args.out, "w") as f:
