@@ -239,13 +239,39 @@ struct VsegOff {
 };
 
 template <typename T>
+int launch_sub(const ChainSub& s, hipStream_t st) {
+  BSeg seg{s.seg, s.seg + s.grid};
+  if (s.nt == kNT) return launch_spec<T, kNT>(s.spec, s.grid, s.lds, st, s.tab, seg);
+  if (s.nt == kNTSmall) return launch_spec<T, kNTSmall>(s.spec, s.grid, s.lds, st, s.tab, seg);
+  return launch_spec<T, kNTBig>(s.spec, s.grid, s.lds, st, s.tab, seg);
+}
+
+template <typename T>
 int run_chain(const Chain& c, hipStream_t st) {
-  for (const ChainSub& s : c.subs) {
-    BSeg seg{s.seg, s.seg + s.grid};
-    if (s.nt == kNT) CHK((launch_spec<T, kNT>(s.spec, s.grid, s.lds, st, s.tab, seg)));
-    else if (s.nt == kNTSmall) CHK((launch_spec<T, kNTSmall>(s.spec, s.grid, s.lds, st, s.tab, seg)));
-    else CHK((launch_spec<T, kNTBig>(s.spec, s.grid, s.lds, st, s.tab, seg)));
-  }
+  for (const ChainSub& s : c.subs) CHK(launch_sub<T>(s, st));
+  return CSIPM_OK;
+}
+
+// the same with the passes of the 1D and 2D instances on `side`, concurrent
+// with those of the 3D instances on `st` (the instances of a launch are
+// independent; each dimensionality keeps its pass order): a fork from `st`
+// and a join back into it, which a graph capture of `st` records as two
+// branches.  The small instances' passes, latency-bound launches of few
+// workgroups, then fill the CUs the 3D passes leave idle.
+template <typename T>
+int run_chain_par(const Chain& c, hipStream_t st, hipStream_t side, hipEvent_t fork_ev,
+                  hipEvent_t join_ev) {
+  bool big = false, small = false;
+  for (const ChainSub& s : c.subs) (s.spec.D == 3 ? big : small) = true;
+  if (!big || !small || !side) return run_chain<T>(c, st);
+  HIPCHK(hipEventRecord(fork_ev, st));
+  HIPCHK(hipStreamWaitEvent(side, fork_ev, 0));
+  for (const ChainSub& s : c.subs)
+    if (s.spec.D != 3) CHK(launch_sub<T>(s, side));
+  HIPCHK(hipEventRecord(join_ev, side));
+  for (const ChainSub& s : c.subs)
+    if (s.spec.D == 3) CHK(launch_sub<T>(s, st));
+  HIPCHK(hipStreamWaitEvent(st, join_ev, 0));
   return CSIPM_OK;
 }
 
@@ -274,6 +300,11 @@ struct Engine {
   hipStream_t stream = nullptr;   // the engine's own stream (all its work)
   hipStream_t user = nullptr;     // the caller's stream, ordered around every call
   hipEvent_t ev = nullptr;
+  // a second stream: an apply's passes of the 1D and 2D instances run on it
+  // beside those of the 3D instances (forked from and joined back into
+  // `stream`, also inside the captured CG chunks)
+  hipStream_t side = nullptr;
+  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
   bool use_graphs = true;         // CG chunks replayed as captured HIP graphs
   // captured CG chunks: [0] four plain iterations, [1] three plain + one with
   // the reliable update; valid for (graph_nb, graph_rel, graph_version)
@@ -356,8 +387,12 @@ struct Engine {
     release_tables();
     // blocks go back to the device-block cache: the stream's work ends first
     if (stream) (void)hipStreamSynchronize(stream);
+    if (side) (void)hipStreamSynchronize(side);
     if (stream) (void)hipStreamDestroy(stream);
+    if (side) (void)hipStreamDestroy(side);
     if (ev) (void)hipEventDestroy(ev);
+    if (fork_ev) (void)hipEventDestroy(fork_ev);
+    if (join_ev) (void)hipEventDestroy(join_ev);
     for (void* p : {(void*)info, (void*)flags, (void*)cg, (void*)scal, (void*)part, (void*)valid,
                     (void*)Wd, Wc})
       dfree(p);
@@ -404,7 +439,10 @@ int engine_create(Engine** out, int64_t cap_slots, int maxinst, int tc_size, boo
       return bad(fail(CSIPM_ERR_ALLOC, "%s failed: %s", #expr, hipGetErrorString(e_))); \
   } while (0)
   ECHK(hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking));
+  ECHK(hipStreamCreateWithFlags(&E->side, hipStreamNonBlocking));
   ECHK(hipEventCreateWithFlags(&E->ev, hipEventDisableTiming));
+  ECHK(hipEventCreateWithFlags(&E->fork_ev, hipEventDisableTiming));
+  ECHK(hipEventCreateWithFlags(&E->join_ev, hipEventDisableTiming));
   ECHK(dmalloc((void**)&E->info, sizeof(BInst) * (size_t)maxinst));
   ECHK(dmalloc((void**)&E->flags, sizeof(BFlags) * (size_t)maxinst));
   ECHK(hmalloc((void**)&E->flags_h, sizeof(BFlags) * (size_t)maxinst));
@@ -914,13 +952,13 @@ int cg_iteration(Engine* E, int nb, bool reliable) {
   TC *z = E->C2<TC>(C2_Z), *p = E->C2<TC>(C2_P), *q = E->C2<TC>(C2_Q);
   TC *P11 = E->C<TC>(C_P11), *P12 = E->C<TC>(C_P12), *P22 = E->C<TC>(C_P22);
   hipStream_t st = E->stream;
-  CHK(run_chain<TC>(E->chains[CALL_CGA], st));
+  CHK(run_chain_par<TC>(E->chains[CALL_CGA], st, E->side, E->fork_ev, E->join_ev));
   hipLaunchKernelGGL((bk_cg_update<TC>), dim3(E->vgrid), dim3(kNT), 0, st, E->vseg, E->info,
                      E->cg, x, r, z, p, q, P11, P12, P22, nb, E->part);
   hipLaunchKernelGGL(bk_cg_gamma, dim3(na), dim3(kNT), 0, st, E->vseg, E->ids, E->info, E->cg,
                      E->part);
   if (reliable) {
-    CHK(run_chain<TC>(E->chains[CALL_CGR], st));
+    CHK(run_chain_par<TC>(E->chains[CALL_CGR], st, E->side, E->fork_ev, E->join_ev));
     hipLaunchKernelGGL((bk_cg_residual<TC>), dim3(E->vgrid), dim3(kNT), 0, st, E->vseg, E->info,
                        E->cg, b, q, r, z, P11, P12, P22, nb, E->part);
     hipLaunchKernelGGL(bk_cg_regamma, dim3(na), dim3(kNT), 0, st, E->vseg, E->ids, E->info,
@@ -1035,7 +1073,7 @@ int engine_newton(Engine* E, double atol, double rtol, int reliable, bool refine
   if (refine) {
     hipLaunchKernelGGL((bk_convert<double, double>), dim3(E->vgrid), dim3(kNT), 0, st, sg,
                        E->info, E->flags, E->W2(V2_RHSD), E->W2(V2_TMP), 1);
-    CHK(run_chain<double>(E->chains[CALL_REF], st));
+    CHK(run_chain_par<double>(E->chains[CALL_REF], st, E->side, E->fork_ev, E->join_ev));
     hipLaunchKernelGGL((bk_convert<double, TC>), dim3(E->vgrid), dim3(kNT), 0, st, sg, E->info,
                        E->flags, E->W2(V2_TMP), E->C2<TC>(C2_RHS), 1);
     HIPCHK(hipGetLastError());
