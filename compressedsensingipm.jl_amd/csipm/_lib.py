@@ -33,6 +33,7 @@ SIGNATURES = {
                                 ctypes.POINTER(_i)]),
     "csipm_tune_set": (_i, [_i, _i, _i, _i, _i, _i]),
     "csipm_tune_clear": (_i, []),
+    "csipm_tune_bluestein": (_i, [_i, _i]),
     "csipm_op_bytes": (_i, [_vp, ctypes.POINTER(_i64)]),
     "csipm_op_footprint": (_i, [_i, ctypes.POINTER(_i64), _i, _i64, ctypes.POINTER(_i64),
                                 ctypes.POINTER(_i64)]),

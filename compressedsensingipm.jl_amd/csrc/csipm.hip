@@ -426,10 +426,15 @@ static bool is_prime(int n) {
   return true;
 }
 
+// Bluestein convolution lengths chosen by measurement (csipm_tune_bluestein):
+// line length -> M.  A length without an entry takes the smallest M.
+static std::mutex g_blue_mu;
+static std::map<int, int> g_blue_m;
+
 // axis plan: the direct mixed-radix plan when L factors into the supported
 // radices; for a prime L whose L - 1 does, Rader's form (plan of length
-// L - 1); else Bluestein with the smallest convolution length M >= 2L - 1
-// that factors.  rows: LDS rows a line needs.
+// L - 1); else Bluestein with the tuned convolution length of L, or the
+// smallest M >= 2L - 1 that factors.  rows: LDS rows a line needs.
 static bool plan_axis(int L, FftPlan& p, int& bs, int& rows) {
   bs = 0;
   rows = L;
@@ -440,6 +445,16 @@ static bool plan_axis(int L, FftPlan& p, int& bs, int& rows) {
     return true;
   }
   bs = 1;
+  int tuned = 0;
+  {
+    std::lock_guard<std::mutex> lk(g_blue_mu);
+    const auto it = g_blue_m.find(L);
+    if (it != g_blue_m.end()) tuned = it->second;
+  }
+  if (tuned && factorize(tuned, p)) {
+    rows = tuned;
+    return true;
+  }
   for (int M = 2 * L - 1; M <= 4 * L; ++M)
     if (factorize(M, p)) {
       rows = M;
@@ -2071,8 +2086,28 @@ int csipm_tune_set(int kind, int elem_bytes, int fft_len, int rows, int lg_cap, 
 }
 
 int csipm_tune_clear(void) {
-  std::lock_guard<std::mutex> lk(g_tune_mu);
-  g_tune.clear();
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune.clear();
+  }
+  std::lock_guard<std::mutex> lk(g_blue_mu);
+  g_blue_m.clear();
+  return CSIPM_OK;
+}
+
+int csipm_tune_bluestein(int line_len, int conv_len) {
+  if (line_len < 3 || line_len > 16384) return fail(CSIPM_ERR_ARG, "bad line length %d", line_len);
+  std::lock_guard<std::mutex> lk(g_blue_mu);
+  if (conv_len == 0) {
+    g_blue_m.erase(line_len);
+    return CSIPM_OK;
+  }
+  FftPlan p;
+  if (conv_len < 2 * line_len - 1 || conv_len > 4 * line_len || !factorize(conv_len, p))
+    return fail(CSIPM_ERR_ARG,
+                "Bluestein length %d does not serve line length %d (needs 2L - 1 <= M <= 4L "
+                "and a factorable M)", conv_len, line_len);
+  g_blue_m[line_len] = conv_len;
   return CSIPM_OK;
 }
 

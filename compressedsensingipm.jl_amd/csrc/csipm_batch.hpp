@@ -129,19 +129,45 @@ __global__ __launch_bounds__(256) void bk_cg_init(BSeg seg, const BInst* __restr
   store_partial(part + I.pbase, acc, lb);
 }
 
-// one block per instance: gamma = r'z, eps = atol + rtol sqrt(gamma)
+// warm start of the reduced system (nb = 1): x and its residual r = b - K x
+// are given; z = P r, p = z; partials r'z and, nbk slots further, b'Pb (the
+// stopping test stays relative to the right-hand side)
+template <typename T>
+__global__ __launch_bounds__(256) void bk_cg_init_warm(BSeg seg, const BInst* __restrict__ info,
+                                                       const BFlags* flags,
+                                                       const T* __restrict__ b,
+                                                       const T* __restrict__ r, T* z, T* p,
+                                                       const T* __restrict__ P11, double* part) {
+  BSEG_BEGIN
+  double acc[2] = {0.0, 0.0};
+  BFOR(i) {
+    const int64_t j = i + I.off;
+    const T rb = r[j], bb = b[j], pd = P11[i];
+    const T zb = pd * rb;
+    z[j] = zb;
+    p[j] = zb;
+    acc[0] += (double)rb * zb;
+    acc[1] += (double)bb * (pd * bb);
+  }
+  const int ops[2] = {0, 0};
+  bstore_partials<2>(part + I.pbase, acc, ops, lb, nbk);
+}
+
+// one block per instance: gamma = r'z, eps = atol + rtol sqrt(gamma) (warm:
+// rtol sqrt(b'Pb), from the second partial set)
 __global__ void bk_cg_start(BSeg vseg, const int* ids, const BInst* __restrict__ info,
                             const BFlags* flags, CgState* cg, const double* part, double atol,
-                            double rtol) {
+                            double rtol, int warm) {
   const int inst = ids[blockIdx.x];
   const BInst& I = info[inst];
   const int np = vseg.first[inst + 1] - vseg.first[inst];
   const double g = sum_partials(part + I.pbase, np);
+  const double gb = warm ? sum_partials(part + I.pbase + np, np) : g;
   if (threadIdx.x == 0) {
     CgState& st = cg[inst];
     st.gamma = g;
     st.rnorm = sqrt(g > 0 ? g : 0.0);
-    st.eps = atol + rtol * st.rnorm;
+    st.eps = atol + rtol * sqrt(gb > 0 ? gb : 0.0);
     st.iters = 0;
     st.itmax = I.itmax;
     st.status = 1;
@@ -350,7 +376,9 @@ enum {
 // One block per instance (ids[block]).  mode 0: reductions into scal[first + q]; with
 // first == S_RB also mu, the objective, the dual residual and the stopping
 // test (tol, max_iter); 1: affine step lengths; 2: sigma = min(1,
-// (mu_aff/mu)^3); 3: corrector step lengths with tau.
+// (mu_aff/mu)^3); 3: corrector step lengths with tau; 4: affine step lengths
+// (reductions 0, 1) and sigma from the bilinear form of mu_aff in them
+// (sums 2-5: t'z, t'dz, dt'z, dt'dz).
 __global__ void bk_ipm_finalize(BSeg vseg, const int* ids, const BInst* __restrict__ info,
                                 BFlags* flags, const double* part, RedSpec spec,
                                 double* scal_all, int mode, int first, double tau, double tol,
@@ -362,6 +390,7 @@ __global__ void bk_ipm_finalize(BSeg vseg, const int* ids, const BInst* __restri
   const double* pp = part + I.pbase;
   double* scal = scal_all + (int64_t)inst * S_NSLOTS;
   const double inv2n = 0.5 / (double)I.N;
+  __shared__ double s_sum[4];
   for (int q = 0; q < spec.nq; ++q) {
     const int op = spec.op[q];
     double acc = red_ident(op);
@@ -375,7 +404,8 @@ __global__ void bk_ipm_finalize(BSeg vseg, const int* ids, const BInst* __restri
     else r = block_reduce(acc, OpMin());
     if (threadIdx.x == 0) {
       if (mode == 0) scal[first + q] = r;
-      else if (mode == 1) scal[S_AP_AFF + q] = fmin(1.0, r);
+      else if (mode == 1 || (mode == 4 && q < 2)) scal[S_AP_AFF + q] = fmin(1.0, r);
+      else if (mode == 4) s_sum[q - 2] = r;
       else if (mode == 2) {
         const double ratio = r * inv2n / scal[S_MU];
         scal[S_SIGMA] = fmin(1.0, ratio * ratio * ratio);
@@ -384,6 +414,12 @@ __global__ void bk_ipm_finalize(BSeg vseg, const int* ids, const BInst* __restri
       }
     }
     __syncthreads();
+  }
+  if (mode == 4 && threadIdx.x == 0) {
+    const double ap = scal[S_AP_AFF], ad = scal[S_AD_AFF];
+    const double muaff = (s_sum[0] + ad * s_sum[1] + ap * s_sum[2] + ap * ad * s_sum[3]) * inv2n;
+    const double ratio = muaff / scal[S_MU];
+    scal[S_SIGMA] = fmin(1.0, ratio * ratio * ratio);
   }
   if (mode == 0 && first == S_RB && threadIdx.x == 0) {
     BFlags& fl = flags[inst];
@@ -504,40 +540,46 @@ __global__ __launch_bounds__(256) void bk_ipm_prep(
   }
 }
 
-// Mixed path, the reduced system: L1 = S1 + S2, L2 = S1 - S2 (FP64),
-// D = 4 S1 S2 / (S1 + S2) (FP64, and in TC with the preconditioner
-// 1/(1 + D)), rhsd = [h_b - (L2/L1) h_c; h_c] (FP64, two-block) and its first
-// block in TC (rhs).
+// Mixed path: the residual partials of bk_ipm_resid and, from the same
+// loads, the reduced system (the slack diagonals S1, S2; with L1 = S1 + S2,
+// L2 = S1 - S2, the reduced diagonal D = 4 S1 S2 / L1 in FP64 and in TC with
+// the preconditioner 1/(1 + D); rhsd = [h_b - (L2/L1) h_c; h_c] in FP64 and
+// its first block in TC).  L1 and L2 are not stored: their users form them
+// from S1 and S2.  An instance whose run ends at this step computes the
+// system too, unused.
 template <typename TC>
-__global__ __launch_bounds__(256) void bk_ipm_prep_red(
+__global__ __launch_bounds__(256) void bk_ipm_resid_prep_red(
     BSeg seg, const BInst* __restrict__ info, const BFlags* flags,
     const double* __restrict__ beta, const double* __restrict__ c,
     const double* __restrict__ z1, const double* __restrict__ z2,
-    const double* __restrict__ Qb, const double* __restrict__ b, double* s1o, double* s2o,
-    TC* D, TC* P, TC* rhs, double* L1d, double* L2d, double* Dd, double* rhsd,
-    const uint32_t* valid) {
+    const double* __restrict__ Qb, const double* __restrict__ b, double* part, double* s1o,
+    double* s2o, TC* D, TC* P, TC* rhs, double* Dd, double* rhsd, const uint32_t* valid) {
   BSEG_BEGIN
   const double lam = I.lam;
+  double a[6] = {0, 0, 0, 0, 0, 0};
   BFOR(i) {
-    double s1 = 0, s2 = 0, l1 = 0, l2 = 0, dg = 0, hb = 0, hc = 0;
+    double s1 = 0, s2 = 0, dg = 0, hb = 0, hc = 0;
     if (BVALID(valid, i)) {
-      const double be = beta[i], ce = c[i], y1 = z1[i], y2 = z2[i];
+      const double be = beta[i], ce = c[i], y1 = z1[i], y2 = z2[i], q = Qb[i], bb = b[i];
       const double t1 = ce + be, t2 = ce - be;
+      const double rb = q - bb - y1 + y2;
+      const double rc = lam - y1 - y2;
+      a[0] = fmax(a[0], fabs(rb));
+      a[1] = fmax(a[1], fabs(rc));
+      a[2] += t1 * y1 + t2 * y2;
+      a[3] += be * q;
+      a[4] += be * bb;
+      a[5] += ce;
       s1 = y1 / t1;
       s2 = y2 / t2;
-      l1 = s1 + s2;
-      l2 = s1 - s2;
+      const double l1 = s1 + s2, l2 = s1 - s2;
       dg = 4.0 * s1 * s2 / l1;
-      const double rb = Qb[i] - b[i] - y1 + y2;
-      const double rc = lam - y1 - y2;
       hb = -rb - y1 + y2;
       hc = -rc - y1 - y2;
       hb -= (l2 / l1) * hc;
     }
     s1o[i] = s1;
     s2o[i] = s2;
-    L1d[i] = l1;
-    L2d[i] = l2;
     Dd[i] = dg;
     D[i] = (TC)dg;
     P[i] = (TC)(1.0 / (1.0 + dg));
@@ -546,20 +588,8 @@ __global__ __launch_bounds__(256) void bk_ipm_prep_red(
     rhsd[j + I.n] = hc;
     rhs[j] = (TC)hb;
   }
-}
-
-// dc = (h_c - L2 db) / L1 into the second block of d (FP64)
-__global__ __launch_bounds__(256) void bk_red_back(BSeg seg, const BInst* __restrict__ info,
-                                                   const BFlags* flags, double* d,
-                                                   const double* __restrict__ rhsd,
-                                                   const double* __restrict__ L1d,
-                                                   const double* __restrict__ L2d,
-                                                   const uint32_t* valid) {
-  BSEG_BEGIN
-  BFOR(i) {
-    const int64_t j = i + I.off;
-    d[j + I.n] = BVALID(valid, i) ? (rhsd[j + I.n] - L2d[i] * d[j]) / L1d[i] : 0.0;
-  }
+  const int ops[6] = {1, 1, 0, 0, 0, 0};
+  bstore_partials<6>(part + I.pbase, a, ops, lb, nbk);
 }
 
 // y = x (first nb blocks), with the precision change TI -> TO
@@ -641,6 +671,77 @@ __global__ __launch_bounds__(256) void bk_ipm_dir(
   bstore_partials<2>(part + I.pbase, a, ops, lb, nbk);
 }
 
+// Mixed path direction: the c-block of the reduced system's solution by the
+// back-substitution dc = (h_c - L2 db) / L1 (db = x, h_c the second block of
+// rhsd, L1 = S1 + S2, L2 = S1 - S2), then dt_i, dz_i and the largest steps to
+// the boundary as bk_ipm_dir.  corr = 0: x is the predictor's CG solution
+// (TX = TC) and the affine dt, dz are written; corr = 1: x is the refined
+// corrector direction (TX = double) and dz and dc (into the second block of
+// dco, for the update, which takes db and dc) are written, dt is not.  The
+// affine pass also leaves the sums t'z, t'dz, dt'z, dt'dz (partials 2-5), of
+// which the complementarity after the affine step is the bilinear form
+// mu_aff 2N = t'z + a_d t'dz + a_p dt'z + a_p a_d dt'dz (finalize mode 4).
+template <typename TX>
+__global__ __launch_bounds__(256) void bk_ipm_dir_red(
+    BSeg seg, const BInst* __restrict__ info, const BFlags* flags,
+    const double* __restrict__ beta, const double* __restrict__ c,
+    const double* __restrict__ z1, const double* __restrict__ z2,
+    const double* __restrict__ s1, const double* __restrict__ s2, const TX* x,
+    const double* __restrict__ rhsd, const double* scal_all, int corr, double* dt1o,
+    double* dt2o, double* dz1o, double* dz2o, double* dco, double* part,
+    const uint32_t* valid) {
+  BSEG_BEGIN
+  const double* scal = scal_all + (int64_t)inst * S_NSLOTS;
+  const double smu = corr ? scal[S_SIGMA] * scal[S_MU] : 0.0;
+  double a[6] = {1e300, 1e300, 0.0, 0.0, 0.0, 0.0};
+  BFOR(i) {
+    const int64_t j = i + I.off;
+    if (!BVALID(valid, i)) {
+      dz1o[i] = dz2o[i] = 0.0;
+      if (corr) dco[j + I.n] = 0.0;
+      else dt1o[i] = dt2o[i] = 0.0;
+      continue;
+    }
+    const double be = beta[i], ce = c[i], y1 = z1[i], y2 = z2[i];
+    const double sa = s1[i], sb = s2[i];
+    const double t1 = ce + be, t2 = ce - be;
+    double g1 = -y1, g2 = -y2;
+    if (corr) {
+      g1 = (smu - t1 * y1 - dt1o[i] * dz1o[i]) / t1;
+      g2 = (smu - t2 * y2 - dt2o[i] * dz2o[i]) / t2;
+    }
+    const double db = (double)x[j];
+    const double dc = (rhsd[j + I.n] - (sa - sb) * db) / (sa + sb);
+    const double dt1 = dc + db, dt2 = dc - db;
+    const double dz1 = g1 - sa * dt1;
+    const double dz2 = g2 - sb * dt2;
+    if (corr) {
+      dco[j + I.n] = dc;
+    } else {
+      dt1o[i] = dt1;
+      dt2o[i] = dt2;
+      a[2] += t1 * y1 + t2 * y2;
+      a[3] += t1 * dz1 + t2 * dz2;
+      a[4] += dt1 * y1 + dt2 * y2;
+      a[5] += dt1 * dz1 + dt2 * dz2;
+    }
+    dz1o[i] = dz1;
+    dz2o[i] = dz2;
+    if (dt1 < 0) a[0] = fmin(a[0], -t1 / dt1);
+    if (dt2 < 0) a[0] = fmin(a[0], -t2 / dt2);
+    if (dz1 < 0) a[1] = fmin(a[1], -y1 / dz1);
+    if (dz2 < 0) a[1] = fmin(a[1], -y2 / dz2);
+  }
+  if (corr) {
+    const double b2[2] = {a[0], a[1]};
+    const int ops[2] = {2, 2};
+    bstore_partials<2>(part + I.pbase, b2, ops, lb, nbk);
+  } else {
+    const int ops[6] = {2, 2, 0, 0, 0, 0};
+    bstore_partials<6>(part + I.pbase, a, ops, lb, nbk);
+  }
+}
+
 // complementarity after the affine step
 __global__ __launch_bounds__(256) void bk_ipm_muaff(
     BSeg seg, const BInst* __restrict__ info, const BFlags* flags,
@@ -663,8 +764,8 @@ __global__ __launch_bounds__(256) void bk_ipm_muaff(
   bstore_partials<1>(part + I.pbase, a, ops, lb, nbk);
 }
 
-// corrector right-hand side.  rhsd == null: condensed system, two-block rhs in
-// TC; else the reduced system (see bk_ipm_prep_red).
+// corrector right-hand side of the condensed system (FP64 path), two-block,
+// in TC
 template <typename TC>
 __global__ __launch_bounds__(256) void bk_ipm_rhs_corr(
     BSeg seg, const BInst* __restrict__ info, const BFlags* flags,
@@ -673,16 +774,14 @@ __global__ __launch_bounds__(256) void bk_ipm_rhs_corr(
     const double* __restrict__ Qb, const double* __restrict__ b,
     const double* __restrict__ dt1, const double* __restrict__ dt2,
     const double* __restrict__ dz1, const double* __restrict__ dz2, const double* scal_all,
-    TC* rhs, double* rhsd, const double* __restrict__ L1d, const double* __restrict__ L2d,
-    const uint32_t* valid) {
+    TC* rhs, const uint32_t* valid) {
   BSEG_BEGIN
   const double* scal = scal_all + (int64_t)inst * S_NSLOTS;
   const double smu = scal[S_SIGMA] * scal[S_MU];
   const double lam = I.lam;
   BFOR(i) {
     double hb = 0, hc = 0;
-    const bool ok = BVALID(valid, i);
-    if (ok) {
+    if (BVALID(valid, i)) {
       const double be = beta[i], ce = c[i], y1 = z1[i], y2 = z2[i];
       const double t1 = ce + be, t2 = ce - be;
       const double g1 = (smu - t1 * y1 - dt1[i] * dz1[i]) / t1;
@@ -693,15 +792,45 @@ __global__ __launch_bounds__(256) void bk_ipm_rhs_corr(
       hc = -rc + g1 + g2;
     }
     const int64_t j = i + I.off;
-    if (rhsd) {
-      if (ok) hb -= (L2d[i] / L1d[i]) * hc;
-      rhsd[j] = hb;
-      rhsd[j + I.n] = hc;
-      rhs[j] = (TC)hb;
-    } else {
-      rhs[j] = (TC)hb;
-      rhs[j + I.n] = (TC)hc;
-    }
+    rhs[j] = (TC)hb;
+    rhs[j + I.n] = (TC)hc;
+  }
+}
+
+// Mixed path corrector right-hand side, from the predictor's: the
+// corrector's g_i = (sigma mu - t_i z_i - dt_i dz_i) / t_i (affine dt, dz)
+// exceed the predictor's g_i = -z_i by delta_i = (sigma mu - dt_i dz_i) / t_i,
+// so h_b grows by delta_1 - delta_2 and h_c by delta_1 + delta_2: rhsd (the
+// predictor's reduced right-hand side on entry) gets
+// [dh_b - (L2/L1) dh_c; dh_c] added, rhs is its new first block in TC.
+// r (when given) holds the predictor CG's final residual h_b - K db and gets
+// dh_b added: the residual of the predictor's db for the corrector's system,
+// the corrector CG's warm start.  Invalid slots keep their zeros.
+template <typename TC>
+__global__ __launch_bounds__(256) void bk_ipm_rhs_corr_red(
+    BSeg seg, const BInst* __restrict__ info, const BFlags* flags,
+    const double* __restrict__ beta, const double* __restrict__ c,
+    const double* __restrict__ s1, const double* __restrict__ s2,
+    const double* __restrict__ dt1, const double* __restrict__ dt2,
+    const double* __restrict__ dz1, const double* __restrict__ dz2, const double* scal_all,
+    TC* rhs, double* rhsd, TC* r, const uint32_t* valid) {
+  BSEG_BEGIN
+  const double* scal = scal_all + (int64_t)inst * S_NSLOTS;
+  const double smu = scal[S_SIGMA] * scal[S_MU];
+  BFOR(i) {
+    if (!BVALID(valid, i)) continue;
+    const double be = beta[i], ce = c[i], sa = s1[i], sb = s2[i];
+    const double t1 = ce + be, t2 = ce - be;
+    const double d1 = (smu - dt1[i] * dz1[i]) / t1;
+    const double d2 = (smu - dt2[i] * dz2[i]) / t2;
+    const double dhc = d1 + d2;
+    const double dhb = (d1 - d2) - ((sa - sb) / (sa + sb)) * dhc;
+    const int64_t j = i + I.off;
+    const double hb = rhsd[j] + dhb;
+    rhsd[j] = hb;
+    rhsd[j + I.n] += dhc;
+    rhs[j] = (TC)hb;
+    if (r) r[j] = (TC)((double)r[j] + dhb);
   }
 }
 

@@ -283,8 +283,8 @@ constexpr int kPartPerAlign = 64;     // partial-sum slots per 256 vector slots 
                                       // over U slots has at most U/4 workgroups)
 
 // FP64 control vectors (one block, pool length) and two-block FP64 vectors
-enum { V_B = 0, V_BETA, V_C, V_Z1, V_Z2, V_QB, V_S1, V_S2, V_DT1, V_DT2, V_DZ1, V_DZ2, V_L1D,
-       V_L2D, V_DGD, V_NV1 };
+enum { V_B = 0, V_BETA, V_C, V_Z1, V_Z2, V_QB, V_S1, V_S2, V_DT1, V_DT2, V_DZ1, V_DZ2, V_DGD,
+       V_NV1 };
 enum { V2_RHSD = 0, V2_DD, V2_TMP, V_NV2 };
 // CG-precision vectors: one block (L1 or D, L2, P11 or P, P12, P22), two-block
 // (rhs, d) and the CG vectors r, z, p, q (two-block)
@@ -1012,17 +1012,21 @@ int capture_cg_graphs(Engine* E, int nb, int reliable) {
 // kCgChunk iterations; the chunks in between are replayed HIP graphs.
 // ---------------------------------------------------------------------------
 template <typename TC>
-int engine_cg(Engine* E, int nb, double atol, double rtol, int reliable) {
+int engine_cg(Engine* E, int nb, double atol, double rtol, int reliable, bool warm = false) {
   StatTimer tm(&E->stats[Engine::ST_CG]);
   const int na = (int)E->active.size();
   TC *b = E->C2<TC>(C2_RHS), *x = E->C2<TC>(C2_D), *r = E->C2<TC>(C2_R);
   TC *z = E->C2<TC>(C2_Z), *p = E->C2<TC>(C2_P);
   TC *P11 = E->C<TC>(C_P11), *P12 = E->C<TC>(C_P12), *P22 = E->C<TC>(C_P22);
   hipStream_t st = E->stream;
-  hipLaunchKernelGGL((bk_cg_init<TC>), dim3(E->vgrid), dim3(kNT), 0, st, E->vseg, E->info,
-                     E->flags, b, x, r, z, p, P11, P12, P22, nb, E->part);
+  if (warm)
+    hipLaunchKernelGGL((bk_cg_init_warm<TC>), dim3(E->vgrid), dim3(kNT), 0, st, E->vseg, E->info,
+                       E->flags, b, r, z, p, P11, E->part);
+  else
+    hipLaunchKernelGGL((bk_cg_init<TC>), dim3(E->vgrid), dim3(kNT), 0, st, E->vseg, E->info,
+                       E->flags, b, x, r, z, p, P11, P12, P22, nb, E->part);
   hipLaunchKernelGGL(bk_cg_start, dim3(na), dim3(kNT), 0, st, E->vseg, E->ids, E->info, E->flags,
-                     E->cg, E->part, atol, rtol);
+                     E->cg, E->part, atol, rtol, warm ? 1 : 0);
   HIPCHK(hipGetLastError());
   bool graphs = E->use_graphs;
   if (graphs && (E->graph_version != E->tables_version || E->graph_nb != nb ||
@@ -1057,33 +1061,35 @@ int engine_cg(Engine* E, int nb, double atol, double rtol, int reliable) {
 // solved by FP32 CG with the diagonal preconditioner 1/(1 + D); the final
 // direction of each step (the corrector, refine = true) gets one FP64
 // refinement step: residual in FP64, correction by FP32 CG to kRefineRtol of
-// that residual.  Direction in V2_DD (FP64).  In FP32 the condensed K itself
-// is unusable near the solution: L1 and |L2| agree to more than 24 bits there.
+// that residual.  db: the predictor's in C2_D (TC), the refined corrector's in
+// V2_DD (FP64); the direction kernel back-substitutes dc.  In FP32 the
+// condensed K itself is unusable near the solution: L1 and |L2| agree to
+// more than 24 bits there.
 // ---------------------------------------------------------------------------
 constexpr double kRefineRtol = 1e-2;
+// the corrector's CG starts from the predictor's solution, whose residual
+// for the corrector's system the right-hand side kernel forms
+constexpr bool kWarmCorrector = true;
 
 template <typename TC>
 int engine_newton(Engine* E, double atol, double rtol, int reliable, bool refine) {
   if (!E->mixed) return engine_cg<TC>(E, 2, atol, rtol, 0);
   hipStream_t st = E->stream;
   const BSeg& sg = E->vseg;
-  CHK(engine_cg<TC>(E, 1, 0.0, rtol, reliable));
+  CHK(engine_cg<TC>(E, 1, 0.0, rtol, reliable, refine && kWarmCorrector));
+  if (!refine) return CSIPM_OK;
   hipLaunchKernelGGL((bk_convert<TC, double>), dim3(E->vgrid), dim3(kNT), 0, st, sg, E->info,
                      E->flags, E->C2<TC>(C2_D), E->W2(V2_DD), 1);
-  if (refine) {
-    hipLaunchKernelGGL((bk_convert<double, double>), dim3(E->vgrid), dim3(kNT), 0, st, sg,
-                       E->info, E->flags, E->W2(V2_RHSD), E->W2(V2_TMP), 1);
-    CHK(run_chain_par<double>(E->chains[CALL_REF], st, E->side, E->fork_ev, E->join_ev));
-    hipLaunchKernelGGL((bk_convert<double, TC>), dim3(E->vgrid), dim3(kNT), 0, st, sg, E->info,
-                       E->flags, E->W2(V2_TMP), E->C2<TC>(C2_RHS), 1);
-    HIPCHK(hipGetLastError());
-    CHK(engine_cg<TC>(E, 1, 0.0, kRefineRtol, reliable));
-    hipLaunchKernelGGL((bk_refine_add<TC>), dim3(E->vgrid), dim3(kNT), 0, st, sg, E->info,
-                       E->flags, E->W2(V2_DD), E->C2<TC>(C2_D), E->W2(V2_TMP),
-                       E->C2<TC>(C2_RHS), E->C2<TC>(C2_R), E->C<TC>(C_L1));
-  }
-  hipLaunchKernelGGL(bk_red_back, dim3(E->vgrid), dim3(kNT), 0, st, sg, E->info, E->flags,
-                     E->W2(V2_DD), E->W2(V2_RHSD), E->W(V_L1D), E->W(V_L2D), E->valid);
+  hipLaunchKernelGGL((bk_convert<double, double>), dim3(E->vgrid), dim3(kNT), 0, st, sg, E->info,
+                     E->flags, E->W2(V2_RHSD), E->W2(V2_TMP), 1);
+  CHK(run_chain_par<double>(E->chains[CALL_REF], st, E->side, E->fork_ev, E->join_ev));
+  hipLaunchKernelGGL((bk_convert<double, TC>), dim3(E->vgrid), dim3(kNT), 0, st, sg, E->info,
+                     E->flags, E->W2(V2_TMP), E->C2<TC>(C2_RHS), 1);
+  HIPCHK(hipGetLastError());
+  CHK(engine_cg<TC>(E, 1, 0.0, kRefineRtol, reliable));
+  hipLaunchKernelGGL((bk_refine_add<TC>), dim3(E->vgrid), dim3(kNT), 0, st, sg, E->info,
+                     E->flags, E->W2(V2_DD), E->C2<TC>(C2_D), E->W2(V2_TMP),
+                     E->C2<TC>(C2_RHS), E->C2<TC>(C2_R), E->C<TC>(C_L1));
   HIPCHK(hipGetLastError());
   return CSIPM_OK;
 }
@@ -1123,9 +1129,17 @@ int engine_step(Engine* E, double tol, int max_iter, double cg_atol, double cg_r
   // M^T M d from the corrector's FP64 refinement apply (its uncorrected
   // part) and its refinement CG (the correction's part, b - r - D x in FP32:
   // the correction is a 1e-5 part of d, so FP64-level accuracy)
+  TC* rhs = E->C2<TC>(C2_RHS);
+  double* rhsd = E->mixed ? E->W2(V2_RHSD) : nullptr;
   if (!E->mixed) CHK(run_chain<double>(E->chains[CALL_QB], st));
-  hipLaunchKernelGGL(bk_ipm_resid, dim3(g), dim3(kNT), 0, st, sg, E->info, E->flags, beta, c, z1,
-                     z2, Qb, b, E->part, E->valid);
+  // mixed: the reduced system comes from the residual kernel's own loads
+  if (E->mixed)
+    hipLaunchKernelGGL((bk_ipm_resid_prep_red<TC>), dim3(g), dim3(kNT), 0, st, sg, E->info,
+                       E->flags, beta, c, z1, z2, Qb, b, E->part, s1, s2, E->C<TC>(C_L1),
+                       E->C<TC>(C_P11), rhs, E->W(V_DGD), rhsd, E->valid);
+  else
+    hipLaunchKernelGGL(bk_ipm_resid, dim3(g), dim3(kNT), 0, st, sg, E->info, E->flags, beta, c,
+                       z1, z2, Qb, b, E->part, E->valid);
   CHK(finalize(E, sg, E->ids, na, {1, 1, 0, 0, 0, 0}, 0, S_RB, tau, tol, max_iter));
   HIPCHK(hipMemcpyAsync(E->flags_h, E->flags, sizeof(BFlags) * (size_t)E->maxinst,
                         hipMemcpyDeviceToHost, st));
@@ -1134,13 +1148,7 @@ int engine_step(Engine* E, double tol, int max_iter, double cg_atol, double cg_r
   for (int id : E->active) run += E->flags_h[id].done ? 0 : 1;
   *running = run;
   if (run == 0) return CSIPM_OK;
-  TC* rhs = E->C2<TC>(C2_RHS);
-  double* rhsd = E->mixed ? E->W2(V2_RHSD) : nullptr;
-  if (E->mixed)
-    hipLaunchKernelGGL((bk_ipm_prep_red<TC>), dim3(g), dim3(kNT), 0, st, sg, E->info, E->flags,
-                       beta, c, z1, z2, Qb, b, s1, s2, E->C<TC>(C_L1), E->C<TC>(C_P11), rhs,
-                       E->W(V_L1D), E->W(V_L2D), E->W(V_DGD), rhsd, E->valid);
-  else
+  if (!E->mixed)
     hipLaunchKernelGGL((bk_ipm_prep<TC>), dim3(g), dim3(kNT), 0, st, sg, E->info, E->flags, beta,
                        c, z1, z2, Qb, b, s1, s2, E->C<TC>(C_L1), E->C<TC>(C_L2),
                        E->C<TC>(C_P11), E->C<TC>(C_P12), E->C<TC>(C_P22), rhs, E->valid);
@@ -1148,10 +1156,14 @@ int engine_step(Engine* E, double tol, int max_iter, double cg_atol, double cg_r
   // predictor
   CHK(engine_newton<TC>(E, atol, rtol, reliable, false));
   auto dir = [&](int corr) -> int {
-    if (E->mixed)
-      hipLaunchKernelGGL((bk_ipm_dir<double>), dim3(g), dim3(kNT), 0, st, sg, E->info, E->flags,
-                         beta, c, z1, z2, s1, s2, E->W2(V2_DD), E->scal, corr, dt1, dt2, dz1, dz2,
-                         E->part, E->valid);
+    if (E->mixed && corr)
+      hipLaunchKernelGGL((bk_ipm_dir_red<double>), dim3(g), dim3(kNT), 0, st, sg, E->info,
+                         E->flags, beta, c, z1, z2, s1, s2, E->W2(V2_DD), rhsd, E->scal, 1, dt1,
+                         dt2, dz1, dz2, E->W2(V2_DD), E->part, E->valid);
+    else if (E->mixed)
+      hipLaunchKernelGGL((bk_ipm_dir_red<TC>), dim3(g), dim3(kNT), 0, st, sg, E->info, E->flags,
+                         beta, c, z1, z2, s1, s2, E->C2<TC>(C2_D), rhsd, E->scal, 0, dt1, dt2,
+                         dz1, dz2, nullptr, E->part, E->valid);
     else
       hipLaunchKernelGGL((bk_ipm_dir<TC>), dim3(g), dim3(kNT), 0, st, sg, E->info, E->flags, beta,
                          c, z1, z2, s1, s2, E->C2<TC>(C2_D), E->scal, corr, dt1, dt2, dz1, dz2,
@@ -1160,14 +1172,23 @@ int engine_step(Engine* E, double tol, int max_iter, double cg_atol, double cg_r
     return CSIPM_OK;
   };
   CHK(dir(0));
-  CHK(finalize(E, sg, E->ids, na, {2, 2}, 1, 0, tau, tol, max_iter));
-  hipLaunchKernelGGL(bk_ipm_muaff, dim3(g), dim3(kNT), 0, st, sg, E->info, E->flags, beta, c, z1,
-                     z2, dt1, dt2, dz1, dz2, E->scal, E->part, E->valid);
-  CHK(finalize(E, sg, E->ids, na, {0}, 2, 0, tau, tol, max_iter));
+  if (E->mixed) {
+    // step lengths and sigma in one pass (the direction kernel's sums)
+    CHK(finalize(E, sg, E->ids, na, {2, 2, 0, 0, 0, 0}, 4, 0, tau, tol, max_iter));
+  } else {
+    CHK(finalize(E, sg, E->ids, na, {2, 2}, 1, 0, tau, tol, max_iter));
+    hipLaunchKernelGGL(bk_ipm_muaff, dim3(g), dim3(kNT), 0, st, sg, E->info, E->flags, beta, c,
+                       z1, z2, dt1, dt2, dz1, dz2, E->scal, E->part, E->valid);
+    CHK(finalize(E, sg, E->ids, na, {0}, 2, 0, tau, tol, max_iter));
+  }
   // corrector
-  hipLaunchKernelGGL((bk_ipm_rhs_corr<TC>), dim3(g), dim3(kNT), 0, st, sg, E->info, E->flags,
-                     beta, c, z1, z2, Qb, b, dt1, dt2, dz1, dz2, E->scal, rhs, rhsd, E->W(V_L1D),
-                     E->W(V_L2D), E->valid);
+  if (E->mixed)
+    hipLaunchKernelGGL((bk_ipm_rhs_corr_red<TC>), dim3(g), dim3(kNT), 0, st, sg, E->info,
+                       E->flags, beta, c, s1, s2, dt1, dt2, dz1, dz2, E->scal, rhs, rhsd,
+                       kWarmCorrector ? E->C2<TC>(C2_R) : nullptr, E->valid);
+  else
+    hipLaunchKernelGGL((bk_ipm_rhs_corr<TC>), dim3(g), dim3(kNT), 0, st, sg, E->info, E->flags,
+                       beta, c, z1, z2, Qb, b, dt1, dt2, dz1, dz2, E->scal, rhs, E->valid);
   HIPCHK(hipGetLastError());
   CHK(engine_newton<TC>(E, atol, rtol, reliable, true));
   CHK(dir(1));

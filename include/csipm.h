@@ -134,7 +134,16 @@ int csipm_op_fft_shape(csipm_op_t h, int axis, int* fft_len, int* rows, int* mod
  * table the package loads at start-up.  Takes effect at the next table
  * rebuild of an engine (a new batch, or the next change of its instances). */
 int csipm_tune_set(int kind, int elem_bytes, int fft_len, int rows, int lg_cap, int nt);
+/* Removes every launch-shape and Bluestein-length entry.                  */
 int csipm_tune_clear(void);
+/* Autotuned Bluestein convolution length (no counterpart in the reference):
+ * 2D/3D axes and 1D lines of length line_len that take Bluestein's form
+ * convolve at length conv_len (2 line_len - 1 <= conv_len <= 4 line_len,
+ * factorable into the supported radices) instead of the smallest such
+ * length; conv_len 0 removes the entry.  Fewer, cheaper radix stages can
+ * outweigh a longer convolution; csipm.tune --bluestein measures the
+ * candidates.  Takes effect for handles created afterwards.              */
+int csipm_tune_bluestein(int line_len, int conv_len);
 
 /* beta -> DFT map.  beta_to_DFT! (src/fft_utils.jl:195-204, per-element form
  * src/mapping_cpu_kernels.jl:353-738).  v: (M1+1)*N2*N3 complex when full==0,

@@ -95,6 +95,38 @@ def test_operator_masked_vs_oracle(cuda, dims, dtype):
     assert _rel(M_perpt_z(o, xt).double().cpu().numpy(), op.m_perpt_z(x, dims)) <= tol
 
 
+# a tuned Bluestein convolution length (csipm_tune_bluestein) changes the
+# plan, not the operator
+@pytest.mark.parametrize("dims,L,M", [((83,), 83, 176), ((111, 20), 111, 240),
+                                      ((9, 8, 83), 83, 171), ((37, 41, 47), 37, 80)])
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_operator_tuned_bluestein(cuda, dims, L, M, dtype):
+    from csipm import _lib, tune
+    from csipm.fft_operator import M_perp_beta, M_perpt_M_perp_vec, M_perpt_z
+    print("tuned Bluestein", dims, L, M, dtype, flush=True)
+    lib = _lib.lib()
+    n = int(np.prod(dims))
+    rng = np.random.default_rng(n + M)
+    miss = np.sort(rng.choice(n, size=n // 9, replace=False))
+    x = rng.standard_normal(n)
+    tol = TOL64 if dtype == torch.float64 else TOL32
+    assert lib.csipm_tune_bluestein(L, M) == 0
+    try:
+        o = _make(dims, miss, dtype)
+        axis = list(dims).index(L)
+        if (L, dims) == (37, (37, 41, 47)):
+            assert tune.fft_shape(o, axis)[2] == 2       # a Rader length keeps Rader's form
+        else:
+            assert tune.fft_shape(o, axis) == (M, M, 1)
+        xt = torch.from_numpy(x).to(cuda, dtype)
+        assert _rel(M_perpt_M_perp_vec(o, xt).double().cpu().numpy(), op.mtm(x, dims, miss)) <= tol
+        assert _rel(M_perp_beta(o, xt).double().cpu().numpy(), op.m_perp_beta(x, dims, miss)) <= tol
+        assert _rel(M_perpt_z(o, xt).double().cpu().numpy(), op.m_perpt_z(x, dims)) <= tol
+    finally:
+        lib.csipm_tune_bluestein(L, 0)
+        tune.load(lib)
+
+
 @pytest.mark.parametrize("dims", [(1024,), (255, 255), (33, 30, 26)])
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
 def test_kkt_and_precond(cuda, dims, dtype):

@@ -27,3 +27,24 @@ def test_library_exports_all_symbols():
     for name in declared():
         assert hasattr(L, name), name
     assert L.csipm_version().decode().count(".") == 2
+
+
+def test_tune_bluestein_arguments():
+    """csipm_tune_bluestein accepts only convolution lengths that serve the
+    line length (2L - 1 <= M <= 4L, factorable) and 0 (remove); host-only."""
+    from csipm import _lib
+    from csipm import tune
+    L = _lib.lib()
+    assert L.csipm_tune_bluestein(111, 240) == 0       # 16 * 15
+    assert L.csipm_tune_bluestein(111, 0) == 0
+    assert L.csipm_tune_bluestein(111, 220) != 0       # below 2L - 1
+    assert L.csipm_tune_bluestein(111, 222) != 0       # 2 * 3 * 37 does not factor
+    assert L.csipm_tune_bluestein(111, 448) != 0       # above 4L
+    assert L.csipm_tune_bluestein(2, 4) != 0
+    tune.load(L)                                        # the committed table again
+    # the tuner's factorisation agrees with the library's on every candidate
+    for m in range(221, 445):
+        ok = L.csipm_tune_bluestein(111, m) == 0
+        assert ok == (tune.radix_stages(m) is not None), m
+    L.csipm_tune_bluestein(111, 0)
+    tune.load(L)
