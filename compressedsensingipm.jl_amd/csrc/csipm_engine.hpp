@@ -312,6 +312,7 @@ struct Engine {
   int graph_nb = 0, graph_rel = -1;
   int64_t graph_version = -1;
   int64_t tables_version = 0;
+  int cg_on_tables = 0;            // CG solves run on the current tables
   std::vector<int64_t> table_sig;  // (id, admission, offset) of the set the tables were built for
   int64_t admissions = 0;
   int table_nb = 0;
@@ -812,6 +813,7 @@ int engine_tables(Engine* E, int nb, bool ipm) {
   E->table_nb = nb;
   E->table_ipm = ipm;
   E->tables_version += 1;
+  E->cg_on_tables = 0;
   return CSIPM_OK;
 }
 
@@ -971,6 +973,12 @@ int cg_iteration(Engine* E, int nb, bool reliable) {
 }
 
 constexpr int kCgChunk = 4;   // CG iterations between host checks (one graph)
+// CG solves on unchanged tables before their chunks are captured as graphs:
+// a capture costs about as much host time as the launches of a step's CG
+// chunks (the GPU idles meanwhile), so tables that change every step (a
+// serving loop admitting and retiring each turn) run their chunks eagerly,
+// and a batch whose instance set holds for a whole step replays graphs
+constexpr int kGraphAfter = 3;
 
 // capture the two CG chunks for the current tables
 template <typename TC>
@@ -1028,7 +1036,8 @@ int engine_cg(Engine* E, int nb, double atol, double rtol, int reliable, bool wa
   hipLaunchKernelGGL(bk_cg_start, dim3(na), dim3(kNT), 0, st, E->vseg, E->ids, E->info, E->flags,
                      E->cg, E->part, atol, rtol, warm ? 1 : 0);
   HIPCHK(hipGetLastError());
-  bool graphs = E->use_graphs;
+  bool graphs = E->use_graphs && E->cg_on_tables >= kGraphAfter;
+  E->cg_on_tables += 1;
   if (graphs && (E->graph_version != E->tables_version || E->graph_nb != nb ||
                  E->graph_rel != reliable))
     CHK(capture_cg_graphs<TC>(E, nb, reliable));
