@@ -17,8 +17,10 @@ sampling mask) is made inside the timed region, as a server must when the
 request arrives.
 
 * value: solves/s of the batched mixed-precision server (csipm.batch.serve:
-  continuous batching, FP32 operator and CG, FP64 control and refinement)
-  at saturation -- all 500*K requests offered at t = 0.
+  continuous batching, FP32 operator and CG, FP64 control and refinement;
+  two engines, one for the 3D requests and one for the 1D/2D ones, each
+  with half the pool, whose interior-point iterations run in parallel
+  threads) at saturation -- all 500*K requests offered at t = 0.
 * plain_fp64: the same requests solved one at a time by csipm_solve on FP64
   handles (FP64 condensed-K PCG, relative 1e-10), at saturation, and
   under the reference's PCG rule (atol 1e-12, rtol 0, src/fft_kkt.jl:339) on
@@ -313,10 +315,14 @@ def run_stream(args):
         return FFTOperator(p.N, len(p.dims), p.dims, p.missing, mixed=True)
 
     # the server's pool (150 GB of the 200 GB budget), made once at start-up
-    # like a serving process's, outside the timed region
+    # like a serving process's, outside the timed region: two engines (lanes),
+    # the 3D requests' and the 1D/2D requests', half the pool each
     from csipm.batch import BatchSolver, pool_bytes
-    server = BatchSolver(int(BUDGET_BYTES * 0.75) // pool_bytes(1, True),
-                         max_instances=args.max_instances, mixed=True)
+    pool_slots = int(BUDGET_BYTES * 0.75) // pool_bytes(1, True)
+    lanes = [(BatchSolver(pool_slots // 2, max_instances=256, mixed=True),
+              lambda q: len(q.dims) == 3),
+             (BatchSolver(pool_slots - pool_slots // 2, max_instances=args.max_instances,
+                          mixed=True), None)]
 
     def batched(idx, arr):
         reqs = [Request(float(arr[j]), k, _Req(probs[k], z0s[k])) for j, k in enumerate(idx)]
@@ -330,7 +336,7 @@ def run_stream(args):
         st = {}
         out = serve(reqs, lambda q: make_mixed(q.p), mixed=True, budget_bytes=BUDGET_BYTES,
                     max_instances=args.max_instances, tol=STREAM_TOL,
-                    prep_workers=args.prep_workers, on_done=progress, stats=st, solver=server)
+                    prep_workers=args.prep_workers, on_done=progress, stats=st, lanes=lanes)
         st["returned_s"] = time.perf_counter() - t_s
         torch.cuda.synchronize()
         st["synced_s"] = time.perf_counter() - t_s
@@ -430,7 +436,9 @@ def run_stream(args):
                                "mixed-precision server",
                    "requests": n_timed, "mix": {f"{d}d": mix[d] for d in mix},
                    "tol": STREAM_TOL, "budget_gb": BUDGET_BYTES >> 30,
-                   "max_instances": args.max_instances},
+                   "max_instances": args.max_instances,
+                   "lanes": "3D engine (half the pool, 256 instances) and 1D/2D engine "
+                            "(half the pool), stepped in parallel threads"},
         "all_solved": solved_b == n_timed and all(r.status == "solved" for r in res_p.values()),
         "not_solved": [[k, list(probs[k].dims), res_b[k].status] for k in res_b
                        if res_b[k].status != "solved"][:10],

@@ -280,7 +280,7 @@ class Request:
 def serve(requests, make_op, mixed=True, budget_bytes=200 << 30, pool_fraction=0.75,
           max_instances=512, tol=1e-6, max_iter=200, cg_rtol=1e-10, cg_atol=0.0,
           clock=time.perf_counter, on_done=None, prep_workers=4, scan=64, stats=None,
-          solver=None):
+          solver=None, lanes=None):
     """Continuous batching over an arrival stream.
 
     requests: iterable of Request (arrival time in seconds from the start,
@@ -306,21 +306,38 @@ def serve(requests, make_op, mixed=True, budget_bytes=200 << 30, pool_fraction=0
     solver: an empty BatchSolver of the same precision to serve with (a
     server's pool, made once at start-up and kept across calls; its slots
     replace pool_fraction * budget_bytes, the handles get the rest of the
-    budget); by default serve makes one and frees it on return."""
+    budget); by default serve makes one and frees it on return.
+
+    lanes: instead of `solver`, several such engines, [(solver, accepts),
+    ...] with accepts(problem) -> bool (None: any problem); a request goes
+    to the first lane that accepts it.  Each turn the lanes' interior-point
+    iterations run in parallel threads (the library releases the GIL), so
+    one engine's host work (synchronisations, launch-table builds) overlaps
+    the other's kernels, and the lanes' vector kernels run on their own
+    streams: a lane of 3D problems and one of 1D/2D problems keep the small
+    problems' vector work off the 3D problems' stream."""
     from collections import deque
     from concurrent.futures import FIRST_COMPLETED, ThreadPoolExecutor
     from concurrent.futures import wait as fwait
     per_slot = pool_bytes(1, mixed)
+    if lanes is not None and solver is not None:
+        raise ValueError("serve takes a solver or lanes, not both")
     if solver is not None:
-        if solver.mixed != bool(mixed) or solver.live:
-            raise ValueError("serve needs an empty solver of the requested precision")
-        pool_slots = solver.slots
-    else:
-        pool_slots = max(1, int(budget_bytes * pool_fraction) // per_slot)
+        lanes = [(solver, None)]
+    if lanes is not None:
+        lanes = [(s, acc) for s, acc in lanes]
+        if not lanes:
+            raise ValueError("serve needs at least one lane")
+        for s, _ in lanes:
+            if s.mixed != bool(mixed) or s.live:
+                raise ValueError("serve needs empty solvers of the requested precision")
+    solvers = [s for s, _ in lanes] if lanes is not None else []
+    pool_slots = sum(s.slots for s in solvers) if solvers else \
+        max(1, int(budget_bytes * pool_fraction) // per_slot)
     handle_budget = budget_bytes - pool_slots * per_slot
     pending = list(requests)
     heapq.heapify(pending)
-    waiting = deque()   # arrived, not admitted: [req, future or op, bytes held]
+    waiting = deque()   # arrived, not admitted: [req, future or op, bytes held, lane]
     nsub = 0            # waiting requests (a prefix) whose handles are made or in the making
     out = []
     held = 0            # handle bytes of waiting (prepared) and live requests
@@ -346,13 +363,23 @@ def serve(requests, make_op, mixed=True, budget_bytes=200 << 30, pool_fraction=0
             on_done(r)
 
     prep = ThreadPoolExecutor(max_workers=prep_workers) if prep_workers > 0 else None
-    bs = solver if solver is not None else \
-        BatchSolver(pool_slots, max_instances=max_instances, mixed=mixed)
+    own = not solvers
+    if own:
+        solvers = [BatchSolver(pool_slots, max_instances=max_instances, mixed=mixed)]
+        lanes = [(solvers[0], None)]
+    stepper = ThreadPoolExecutor(max_workers=len(solvers)) if len(solvers) > 1 else None
+
+    def lane_of(problem):
+        for k, (_, acc) in enumerate(lanes):
+            if acc is None or acc(problem):
+                return k
+        return -1
+
     t_start = clock()
     acc = {"step_s": 0.0, "admit_s": 0.0, "harvest_s": 0.0, "idle_s": 0.0, "steps": 0,
            "live_sum": 0}
     try:
-        while pending or waiting or bs.live:
+        while pending or waiting or any(s.live for s in solvers):
             now = clock() - t_start
             while pending and pending[0].arrival <= now:
                 req = heapq.heappop(pending)
@@ -362,11 +389,15 @@ def serve(requests, make_op, mixed=True, budget_bytes=200 << 30, pool_fraction=0
                 except _lib.CsipmError as e:
                     finish(_rejected(req, str(e), now))
                     continue
-                if slots > pool_slots or hb > handle_budget:
-                    finish(_rejected(req, f"needs {slots} pool slots (pool {pool_slots}) and "
-                                          f"{hb} handle bytes (share {handle_budget})", now))
+                k = lane_of(p)
+                if k < 0:
+                    finish(_rejected(req, "no lane takes this problem", now))
                     continue
-                waiting.append([req, None, hb])
+                if slots > solvers[k].slots or hb > handle_budget:
+                    finish(_rejected(req, f"needs {slots} pool slots (pool {solvers[k].slots}) "
+                                          f"and {hb} handle bytes (share {handle_budget})", now))
+                    continue
+                waiting.append([req, None, hb, k])
             # make handles, in arrival order, while their bytes fit (the
             # first nsub waiting requests have theirs made or in the making)
             for w in itertools.islice(waiting, nsub, None):
@@ -381,14 +412,24 @@ def serve(requests, make_op, mixed=True, budget_bytes=200 << 30, pool_fraction=0
             skipped = 0
             scanned = 0
             cand = []          # (waiting entry, op, z, beta, bytes)
-            room = bs.max_instances - bs.live + scan   # more could not be admitted
+            # more than this could not be admitted to a lane
+            room = [s.max_instances - s.live + scan for s in solvers]
+            ncand = [0] * len(solvers)
+            full = 0           # passed over because their lane has no room
             for w in waiting:
-                if w[1] is None or skipped > scan or len(cand) >= room:
+                if w[1] is None or skipped > scan or full > 8 * scan:
                     break
+                if ncand[w[3]] >= room[w[3]]:
+                    if len(solvers) == 1:
+                        break
+                    scanned += 1
+                    full += 1
+                    continue
                 scanned += 1
                 if not w[1].done():
                     skipped += 1
                     continue
+                ncand[w[3]] += 1
                 try:
                     op, z, beta, nb = w[1].result()
                 except (_lib.CsipmError, ValueError, RuntimeError) as e:
@@ -396,10 +437,13 @@ def serve(requests, make_op, mixed=True, budget_bytes=200 << 30, pool_fraction=0
                     cand.append((w, None, None, None, 0))
                     continue
                 cand.append((w, op, z, beta, nb))
-            todo = [c for c in cand if c[1] is not None]
-            got = bs.admit_many([(op, z, beta, w[0].problem.lam, w[0])
-                                 for w, op, z, beta, _ in todo])
-            res = {id(c[0]): g for c, g in zip(todo, got)}
+            res = {}
+            for k, s in enumerate(solvers):
+                todo = [c for c in cand if c[1] is not None and c[0][3] == k]
+                if todo:
+                    got = s.admit_many([(op, z, beta, w[0].problem.lam, w[0])
+                                        for w, op, z, beta, _ in todo])
+                    res.update({id(c[0]): g for c, g in zip(todo, got)})
             gone = set()
             for w, op, z, beta, nb in cand:
                 req = w[0]
@@ -414,7 +458,7 @@ def serve(requests, make_op, mixed=True, budget_bytes=200 << 30, pool_fraction=0
                         op.close()
                     finish(_rejected(req, str(g), clock() - t_start))
                 elif g is None:
-                    if bs.live == 0:
+                    if solvers[w[3]].live == 0:
                         # an empty engine cannot take it: it never will
                         held -= w[2]
                         gone.add(id(w))
@@ -434,7 +478,8 @@ def serve(requests, make_op, mixed=True, budget_bytes=200 << 30, pool_fraction=0
                         waiting.appendleft(w)
                 nsub -= len(gone)
             acc["admit_s"] += time.perf_counter() - t_a
-            if bs.live == 0:
+            active = [s for s in solvers if s.live]
+            if not active:
                 t_i = time.perf_counter()
                 futs = [w[1] for w in waiting if w[1] is not None and not w[1].done()]
                 nxt = pending[0].arrival - (clock() - t_start) if pending else None
@@ -446,13 +491,19 @@ def serve(requests, make_op, mixed=True, budget_bytes=200 << 30, pool_fraction=0
                 acc["idle_s"] += time.perf_counter() - t_i
                 continue
             t_s = time.perf_counter()
-            acc["live_sum"] += bs.live
-            bs.step(tol, max_iter, cg_rtol, cg_atol)
+            acc["live_sum"] += sum(s.live for s in active)
+            if len(active) == 1:
+                active[0].step(tol, max_iter, cg_rtol, cg_atol)
+            else:
+                for f in [stepper.submit(s.step, tol, max_iter, cg_rtol, cg_atol)
+                          for s in active]:
+                    f.result()
             t_h = time.perf_counter()
             acc["step_s"] += t_h - t_s
             acc["steps"] += 1
             retired = []
-            for r in bs.harvest_many(bs.finished(), retire=retired):
+            for r in itertools.chain.from_iterable(
+                    s.harvest_many(s.finished(), retire=retired) for s in active):
                 r.t_admit -= t_start
                 r.t_done = clock() - t_start
                 held -= live_bytes.pop(r.tag.seq, 0)
@@ -468,14 +519,20 @@ def serve(requests, make_op, mixed=True, budget_bytes=200 << 30, pool_fraction=0
     finally:
         if stats is not None:
             try:
-                stats.update(bs.stats())
+                tot = {}
+                for s in solvers:
+                    for k, v in s.stats().items():
+                        tot[k] = tot.get(k, 0.0) + v
+                stats.update(tot)
             except Exception:
                 pass
             stats.update({"loop_" + k: v for k, v in acc.items()})
             stats["wall_s"] = clock() - t_start
         t_c = time.perf_counter()
-        if solver is None:
-            bs.close()
+        if own:
+            solvers[0].close()
+        if stepper is not None:
+            stepper.shutdown(wait=True)
         if prep is not None:
             prep.shutdown(wait=True, cancel_futures=True)
         if stats is not None:

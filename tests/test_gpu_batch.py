@@ -12,7 +12,8 @@
 * Failure statuses forced from the host: an IPM iteration cap, a CG
   iteration cap of one, a non-finite sample in z0.
 * Continuous batching: problems admitted while others are mid-solve, and
-  retired as they finish, still give the alone results.
+  retired as they finish, still give the alone results; so do serve()'s
+  two-lane form (two engines stepped in parallel threads).
 * The mixed-precision batch reaches the FP64 oracle objective within 1e-6.
 * The CG chunks replayed as captured HIP graphs give bit for bit the result
   of the same launches issued one by one (CSIPM_NO_GRAPH), over the many
@@ -223,6 +224,32 @@ def test_serve_stream(cuda):
     out = serve(reqs, lambda p: _op(p, True), mixed=True, budget_bytes=1 << 30,
                 tol=1e-7, prep_workers=2)
     assert sorted(r.tag.seq for r in out) == list(range(len(probs)))
+    for r in out:
+        p = probs[r.tag.seq]
+        _same(r, _alone(p, True, 1e-7), p.name)
+
+
+def test_serve_lanes(cuda):
+    """serve() over two lanes (a 3D engine and a 1D/2D engine whose
+    interior-point iterations run in parallel threads; pools too small for
+    all at once): every request solved, each result the same as alone."""
+    from csipm.batch import BatchSolver, Request, serve, slots_of
+    print("serve lanes", flush=True)
+    probs = _problems()
+    sl = [slots_of(_op(p, True)) for p in probs]
+    three = [k for k, p in enumerate(probs) if len(p.dims) == 3]
+    rest = [k for k in range(len(probs)) if k not in three]
+    lane3 = BatchSolver(max(sl[k] for k in three), max_instances=2, mixed=True)
+    lane_s = BatchSolver(sum(sorted(sl[k] for k in rest)[-2:]), max_instances=3, mixed=True)
+    reqs = [Request(0.0, k, p) for k, p in enumerate(probs)]
+    st = {}
+    out = serve(reqs, lambda p: _op(p, True), mixed=True, budget_bytes=1 << 30, tol=1e-7,
+                prep_workers=2, stats=st,
+                lanes=[(lane3, lambda p: len(p.dims) == 3), (lane_s, None)])
+    lane3.close()
+    lane_s.close()
+    assert sorted(r.tag.seq for r in out) == list(range(len(probs)))
+    assert st["loop_steps"] > 0
     for r in out:
         p = probs[r.tag.seq]
         _same(r, _alone(p, True, 1e-7), p.name)
