@@ -437,13 +437,31 @@ def serve(requests, make_op, mixed=True, budget_bytes=200 << 30, pool_fraction=0
                     cand.append((w, None, None, None, 0))
                     continue
                 cand.append((w, op, z, beta, nb))
+            todo = [[(op, z, beta, w[0].problem.lam, w[0])
+                     for w, op, z, beta, _ in cand if op is not None and w[3] == k]
+                    for k in range(len(solvers))]
+            keys = [[id(w) for w, op, *_ in cand if op is not None and w[3] == k]
+                    for k in range(len(solvers))]
             res = {}
-            for k, s in enumerate(solvers):
-                todo = [c for c in cand if c[1] is not None and c[0][3] == k]
-                if todo:
-                    got = s.admit_many([(op, z, beta, w[0].problem.lam, w[0])
-                                        for w, op, z, beta, _ in todo])
-                    res.update({id(c[0]): g for c, g in zip(todo, got)})
+            outs = {}
+            t_s = None
+            if len(solvers) == 1:
+                if todo[0]:
+                    res.update(zip(keys[0], solvers[0].admit_many(todo[0])))
+                lane_live = [solvers[0].live]
+            else:
+                # each lane's admissions, interior-point iteration and harvest
+                # in its own thread: one lane's host work overlaps the other's
+                # kernels
+                run = [k for k in range(len(solvers)) if todo[k] or solvers[k].live]
+                t_s = time.perf_counter()
+                futs = [(k, stepper.submit(_lane_turn, solvers[k], todo[k], tol, max_iter,
+                                           cg_rtol, cg_atol)) for k in run]
+                outs = {k: f.result() for k, f in futs}
+                for k, (got, live, fin, ret) in outs.items():
+                    res.update(zip(keys[k], got))
+                lane_live = [outs[k][1] if k in outs else solvers[k].live
+                             for k in range(len(solvers))]
             gone = set()
             for w, op, z, beta, nb in cand:
                 req = w[0]
@@ -458,7 +476,7 @@ def serve(requests, make_op, mixed=True, budget_bytes=200 << 30, pool_fraction=0
                         op.close()
                     finish(_rejected(req, str(g), clock() - t_start))
                 elif g is None:
-                    if solvers[w[3]].live == 0:
+                    if lane_live[w[3]] == 0:
                         # an empty engine cannot take it: it never will
                         held -= w[2]
                         gone.add(id(w))
@@ -477,9 +495,12 @@ def serve(requests, make_op, mixed=True, budget_bytes=200 << 30, pool_fraction=0
                     if id(w) not in gone:
                         waiting.appendleft(w)
                 nsub -= len(gone)
-            acc["admit_s"] += time.perf_counter() - t_a
-            active = [s for s in solvers if s.live]
-            if not active:
+            stepped = [k for k in outs if outs[k][1]]
+            if t_s is None:
+                acc["admit_s"] += time.perf_counter() - t_a
+            else:
+                acc["admit_s"] += t_s - t_a
+            if not stepped and not any(s.live for s in solvers):
                 t_i = time.perf_counter()
                 futs = [w[1] for w in waiting if w[1] is not None and not w[1].done()]
                 nxt = pending[0].arrival - (clock() - t_start) if pending else None
@@ -490,20 +511,23 @@ def serve(requests, make_op, mixed=True, budget_bytes=200 << 30, pool_fraction=0
                     time.sleep(nxt)
                 acc["idle_s"] += time.perf_counter() - t_i
                 continue
-            t_s = time.perf_counter()
-            acc["live_sum"] += sum(s.live for s in active)
-            if len(active) == 1:
-                active[0].step(tol, max_iter, cg_rtol, cg_atol)
+            retired = []
+            if len(solvers) == 1:
+                t_s = time.perf_counter()
+                acc["live_sum"] += solvers[0].live
+                solvers[0].step(tol, max_iter, cg_rtol, cg_atol)
+                t_h = time.perf_counter()
+                fin = solvers[0].harvest_many(solvers[0].finished(), retire=retired)
             else:
-                for f in [stepper.submit(s.step, tol, max_iter, cg_rtol, cg_atol)
-                          for s in active]:
-                    f.result()
-            t_h = time.perf_counter()
+                # (lanes: the turn's wall time counts as step time, harvests included)
+                t_h = time.perf_counter()
+                acc["live_sum"] += sum(outs[k][1] for k in stepped)
+                fin = [r for k in stepped for r in outs[k][2]]
+                for k in stepped:
+                    retired += outs[k][3]
             acc["step_s"] += t_h - t_s
             acc["steps"] += 1
-            retired = []
-            for r in itertools.chain.from_iterable(
-                    s.harvest_many(s.finished(), retire=retired) for s in active):
+            for r in fin:
                 r.t_admit -= t_start
                 r.t_done = clock() - t_start
                 held -= live_bytes.pop(r.tag.seq, 0)
@@ -538,6 +562,20 @@ def serve(requests, make_op, mixed=True, budget_bytes=200 << 30, pool_fraction=0
         if stats is not None:
             stats["shutdown_s"] = time.perf_counter() - t_c
     return out
+
+
+def _lane_turn(s, items, tol, max_iter, cg_rtol, cg_atol):
+    """One serving turn of a lane (run in a lane thread): its admissions, one
+    interior-point iteration of its instances and the finished ones'
+    results.  Returns (admission results, live instances after admission,
+    results, retired handles)."""
+    got = s.admit_many(items) if items else []
+    live = s.live
+    fin, retired = [], []
+    if live:
+        s.step(tol, max_iter, cg_rtol, cg_atol)
+        fin = s.harvest_many(s.finished(), retire=retired)
+    return got, live, fin, retired
 
 
 class _Ready:
