@@ -90,7 +90,6 @@ def punching_optimized(dims, centers):
     indices in the order punching_centering.jl:131-201 emits them."""
     dims = tuple(dims)
     d = len(dims)
-    out = []
     offs = [np.zeros(d, int)]
     for ax in range(d):
         for s in (-1, 1):
@@ -98,14 +97,13 @@ def punching_optimized(dims, centers):
             o[ax] = s
             offs.append(o)
     # the reference loops i2, j2, k2 in increasing order and keeps |offset|_1 <= 1
-    offs = sorted(offs, key=lambda o: tuple(o))
-    strides = np.cumprod((1,) + dims[:-1])
-    for c in centers:
-        for o in offs:
-            p = np.asarray(c) + o
-            if np.all(p >= 1) and np.all(p <= np.asarray(dims)):
-                out.append(int(np.dot(p - 1, strides)))
-    return np.asarray(out, dtype=np.int64)
+    offs = np.asarray(sorted(offs, key=lambda o: tuple(o)), dtype=np.int64)
+    strides = np.cumprod((1,) + dims[:-1]).astype(np.int64)
+    c = np.asarray(centers, dtype=np.int64).reshape(-1, d)
+    # every centre with every offset, centre-major (the loop order above)
+    p = c[:, None, :] + offs[None, :, :]
+    ok = np.all((p >= 1) & (p <= np.asarray(dims, dtype=np.int64)), axis=2)
+    return ((p - 1) @ strides)[ok].astype(np.int64)
 
 
 def reference_example(dims, check=True, seed=1):
