@@ -18,9 +18,9 @@ request arrives.
 
 * value: solves/s of the batched mixed-precision server (csipm.batch.serve:
   continuous batching, FP32 operator and CG, FP64 control and refinement;
-  two engines, one for the 3D requests and one for the 1D/2D ones, each
-  with half the pool, whose interior-point iterations run in parallel
-  threads) at saturation -- all 500*K requests offered at t = 0.
+  three engines, one each for the 3D, 2D and 1D requests, with half, a
+  quarter and a quarter of the pool, whose turns run in parallel threads)
+  at saturation -- all 500*K requests offered at t = 0.
 * plain_fp64: the same requests solved one at a time by csipm_solve on FP64
   handles (FP64 condensed-K PCG, relative 1e-10), at saturation, and
   under the reference's PCG rule (atol 1e-12, rtol 0, src/fft_kkt.jl:339) on
@@ -38,7 +38,11 @@ request arrives.
 
 Other workloads (--workload 2d255 | 1d1024 | 3d255 | uneven) time fixed
 batches: a step solves --batch instances (2d255, 1d1024), one 255^3 instance
-or the 384 uneven instances, batched-mixed vs plain FP64.
+or the 384 uneven instances, batched-mixed vs plain FP64.  --workload
+sweep3d is the reference's CPU-vs-GPU size sweep (benchmarks/cpu_vs_gpu.jl:
+cubes 8^3 ... 560^3 of test/ipm_example_3D.jl's problem): per size the GPU
+FP64 and mixed solve times, the batched FP32 apply's roofline, and the CPU
+FP64 port's apply (and solve, up to 32^3).
 """
 import argparse
 import json
@@ -73,6 +77,9 @@ WORKLOADS = {
     "3d255": ("punched_3d", 1e-8,
               "3D 255^3, 200 modes, r=2 balls on an 8-lattice + 1% dropout, lambda 0.05, "
               "tol 1e-8"),
+    "sweep3d": ("reference_example", 1e-8,
+                "3D cubes 8^3 ... 560^3 (benchmarks/cpu_vs_gpu.jl), the problem of "
+                "test/ipm_example_3D.jl (noise, punched holes, lambda 5), tol 1e-8"),
     "uneven": ("uneven_batch", 1e-6,
                "256 1D (N log-uniform [512, 65536]) + 128 2D (odd side [63, 511]) instances, "
                "10% dropout + gaps/discs, lambda 0.1, tol 1e-6"),
@@ -315,14 +322,17 @@ def run_stream(args):
         return FFTOperator(p.N, len(p.dims), p.dims, p.missing, mixed=True)
 
     # the server's pool (150 GB of the 200 GB budget), made once at start-up
-    # like a serving process's, outside the timed region: two engines (lanes),
-    # the 3D requests' and the 1D/2D requests', half the pool each
+    # like a serving process's, outside the timed region: three engines
+    # (lanes), the 3D, 2D and 1D requests', with half, a quarter and a
+    # quarter of the pool
     from csipm.batch import BatchSolver, pool_bytes
     pool_slots = int(BUDGET_BYTES * 0.75) // pool_bytes(1, True)
     lanes = [(BatchSolver(pool_slots // 2, max_instances=256, mixed=True),
               lambda q: len(q.dims) == 3),
-             (BatchSolver(pool_slots - pool_slots // 2, max_instances=args.max_instances,
-                          mixed=True), None)]
+             (BatchSolver(pool_slots // 4, max_instances=512, mixed=True),
+              lambda q: len(q.dims) == 2),
+             (BatchSolver(pool_slots - pool_slots // 2 - pool_slots // 4,
+                          max_instances=args.max_instances, mixed=True), None)]
 
     def batched(idx, arr):
         reqs = [Request(float(arr[j]), k, _Req(probs[k], z0s[k])) for j, k in enumerate(idx)]
@@ -437,8 +447,9 @@ def run_stream(args):
                    "requests": n_timed, "mix": {f"{d}d": mix[d] for d in mix},
                    "tol": STREAM_TOL, "budget_gb": BUDGET_BYTES >> 30,
                    "max_instances": args.max_instances,
-                   "lanes": "3D engine (half the pool, 256 instances) and 1D/2D engine "
-                            "(half the pool), stepped in parallel threads"},
+                   "lanes": "3D engine (half the pool, 256 instances), 2D engine (a "
+                            "quarter, 512), 1D engine (a quarter); turns in parallel "
+                            "threads"},
         "all_solved": solved_b == n_timed and all(r.status == "solved" for r in res_p.values()),
         "not_solved": [[k, list(probs[k].dims), res_b[k].status] for k in res_b
                        if res_b[k].status != "solved"][:10],
@@ -521,6 +532,84 @@ def cpu_baseline_batch(problem, tol, threads, applies_per_solve=None):
             "sample": f"median of {runs} FP64 oracle solve(s) of instance 1 after 1 warm-up "
                       f"({med:.3f} s, scipy pocketfft workers={threads}, "
                       f"{res.iterations} IPM iterations)"}, f_ref
+
+
+SWEEP3D_SIDES = (8, 16, 32, 64, 96, 128, 192, 256, 384, 512, 560)
+
+
+def run_sweep3d(args):
+    """The reference's CPU-vs-GPU 3D size sweep (benchmarks/cpu_vs_gpu.jl:15-18):
+    per cube, one FP64 and one mixed GPU solve on one handle (csipm_solve;
+    sizes up to 64^3 after an untimed one), the batched FP32 CG apply's
+    launches timed with HIP events (roofline of the dominant one and of the
+    whole apply against 8 TB/s), and on the host the FP64 port's M^T M apply
+    (up to 256^3) and solve (up to 32^3).  One JSON line, not the metric."""
+    import numpy as np
+    import torch
+    import csipm.workloads as wl
+    from csipm.batch import BatchSolver, slots_of
+    from csipm.fft_operator import FFTOperator, FFTParameters
+    from csipm.model import FFTNLPModel, ipm_solve
+    dev = torch.device("cuda:0")
+    tol = WORKLOADS["sweep3d"][1]
+    threads = min(16, os.cpu_count() or 1)
+    rows = []
+    for n in SWEEP3D_SIDES:
+        dims = (n, n, n)
+        p = wl.reference_example(dims, check=False)
+        z = torch.from_numpy(p.z0).to(dev)
+        row = {"side": n, "N": n ** 3, "missing": int(len(p.missing))}
+        for name, kw in (("fp64", {"dtype": torch.float64}), ("mixed", {"mixed": True})):
+            # rdft = true, as the sweep runs the reference (cpu_vs_gpu.jl:9,27)
+            m = FFTNLPModel(FFTParameters(3, dims, z, p.lam, p.missing), rdft=True, **kw)
+            if n <= 64:
+                ipm_solve(m, tol=tol)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            r = ipm_solve(m, tol=tol)
+            torch.cuda.synchronize()
+            row[name] = {"solve_s": round(time.perf_counter() - t0, 4), "status": r.status,
+                         "ipm": r.iterations, "cg": r.cg_iterations,
+                         "objective": r.objective}
+            del m, r
+        row["mixed_obj_rel_gap"] = abs(row["mixed"]["objective"] - row["fp64"]["objective"]) / \
+            max(abs(row["fp64"]["objective"]), 1e-300)
+        op = FFTOperator(p.N, 3, dims, p.missing, mixed=True)
+        bs = BatchSolver(slots_of(op), max_instances=1, mixed=True)
+        bs.admit(op, z, p.lam)
+        bs.apply_timed(2)
+        launches = bs.apply_timed(10)
+        bs.close()
+        op.close()
+        dom = max(launches, key=lambda t: t[1])
+        apply_ms = sum(t[1] for t in launches)
+        row["apply"] = {"ms": round(apply_ms, 4),
+                        "GBps": round(sum(t[2] for t in launches) / (apply_ms * 1e-3) / 1e9, 1),
+                        "dominant": dom[0], "dominant_ms": round(dom[1], 4),
+                        "dominant_frac": round(dom[2] / (dom[1] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        if args.no_cpu_baseline:
+            pass
+        elif n <= 256:
+            from oracle import operator as orc
+            x = np.random.default_rng(0).standard_normal(p.N)
+            orc.mtm(x, dims, p.missing, workers=threads)
+            t0 = time.perf_counter()
+            orc.mtm(x, dims, p.missing, workers=threads)
+            row["cpu_port"] = {"apply_ms": round(1e3 * (time.perf_counter() - t0), 3),
+                               "threads": threads}
+            if n <= 32:
+                med, runs, res = cpu_solve_time(p, tol, threads, runs=1, budget_s=20.0)
+                row["cpu_port"]["solve_s"] = round(med, 4)
+                row["cpu_port"]["ipm"] = res.iterations
+        del z
+        torch.cuda.empty_cache()
+        log(f"sweep3d {n}^3: " + json.dumps(row))
+        rows.append(row)
+    print(json.dumps({"metric": "3D size sweep: solve seconds and apply roofline per cube side",
+                      "unit": "s", "higher_is_better": False, "n_gpus": 1,
+                      "workload": "sweep3d: " + WORKLOADS["sweep3d"][2],
+                      "hbm_peak_GBps": HBM_PEAK_GBS, "sizes": rows}), flush=True)
+    return 0
 
 
 def run_batch_workload(args):
@@ -659,6 +748,8 @@ def main():
         return 2
     if args.workload == "stream":
         return run_stream(args)
+    if args.workload == "sweep3d":
+        return run_sweep3d(args)
     return run_batch_workload(args)
 
 
