@@ -384,11 +384,12 @@ struct Engine {
   ~Engine() {
     for (Inst& in : inst)
       if (in.h) in.h->admitted.fetch_sub(1);
-    release_graphs();
-    release_tables();
-    // blocks go back to the device-block cache: the stream's work ends first
+    // blocks (tables, staging, vectors) go back to the caches, and graphs are
+    // destroyed: the streams' work ends first
     if (stream) (void)hipStreamSynchronize(stream);
     if (side) (void)hipStreamSynchronize(side);
+    release_graphs();
+    release_tables();
     if (stream) (void)hipStreamDestroy(stream);
     if (side) (void)hipStreamDestroy(side);
     if (ev) (void)hipEventDestroy(ev);
