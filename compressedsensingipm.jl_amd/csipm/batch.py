@@ -378,6 +378,8 @@ def serve(requests, make_op, mixed=True, budget_bytes=200 << 30, pool_fraction=0
     t_start = clock()
     acc = {"step_s": 0.0, "admit_s": 0.0, "harvest_s": 0.0, "idle_s": 0.0, "steps": 0,
            "live_sum": 0}
+    closing = []        # helper-thread destructions of retired handles, not yet done
+    drained = False
     try:
         while pending or waiting or any(s.live for s in solvers):
             now = clock() - t_start
@@ -536,10 +538,12 @@ def serve(requests, make_op, mixed=True, budget_bytes=200 << 30, pool_fraction=0
             # next step (destruction synchronises the handle's stream)
             if retired:
                 if prep is not None:
-                    prep.submit(_close_all, retired)
+                    closing = [f for f in closing if not f.done()]
+                    closing.append(prep.submit(_close_all, retired))
                 else:
                     _close_all(retired)
             acc["harvest_s"] += time.perf_counter() - t_h
+        drained = True
     finally:
         if stats is not None:
             try:
@@ -558,6 +562,10 @@ def serve(requests, make_op, mixed=True, budget_bytes=200 << 30, pool_fraction=0
         if stepper is not None:
             stepper.shutdown(wait=True)
         if prep is not None:
+            if drained:
+                # the last turns' retired handles are destroyed before return
+                # (pending handle preparations are cancelled only on an error)
+                fwait(closing)
             prep.shutdown(wait=True, cancel_futures=True)
         if stats is not None:
             stats["shutdown_s"] = time.perf_counter() - t_c
