@@ -978,8 +978,11 @@ constexpr int kCgChunk = 4;   // CG iterations between host checks (one graph)
 // a capture costs about as much host time as the launches of a step's CG
 // chunks (the GPU idles meanwhile), so tables that change every step (a
 // serving loop admitting and retiring each turn) run their chunks eagerly,
-// and a batch whose instance set holds for a whole step replays graphs
+// and a batch whose instance set holds for a whole step replays graphs.  A
+// small batch (under kGraphAnyBlocks vector blocks of 1024 slots) is
+// launch-bound: its chunks are captured at once.
 constexpr int kGraphAfter = 3;
+constexpr int kGraphAnyBlocks = 4096;
 
 // capture the two CG chunks for the current tables
 template <typename TC>
@@ -1037,7 +1040,7 @@ int engine_cg(Engine* E, int nb, double atol, double rtol, int reliable, bool wa
   hipLaunchKernelGGL(bk_cg_start, dim3(na), dim3(kNT), 0, st, E->vseg, E->ids, E->info, E->flags,
                      E->cg, E->part, atol, rtol, warm ? 1 : 0);
   HIPCHK(hipGetLastError());
-  bool graphs = E->use_graphs && E->cg_on_tables >= kGraphAfter;
+  bool graphs = E->use_graphs && (E->cg_on_tables >= kGraphAfter || E->vgrid < kGraphAnyBlocks);
   E->cg_on_tables += 1;
   if (graphs && (E->graph_version != E->tables_version || E->graph_nb != nb ||
                  E->graph_rel != reliable))
