@@ -587,9 +587,7 @@ def run_sweep3d(args):
                         "GBps": round(sum(t[2] for t in launches) / (apply_ms * 1e-3) / 1e9, 1),
                         "dominant": dom[0], "dominant_ms": round(dom[1], 4),
                         "dominant_frac": round(dom[2] / (dom[1] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
-        if args.no_cpu_baseline:
-            pass
-        elif n <= 256:
+        if not args.no_cpu_baseline and n <= 256:
             from oracle import operator as orc
             x = np.random.default_rng(0).standard_normal(p.N)
             orc.mtm(x, dims, p.missing, workers=threads)
